@@ -1,0 +1,226 @@
+/*
+ * p4gpu.h -- C-ABI of libp4gpu.so, the MI355X likelihood engine behind
+ * Tree.calcLogLike() and the Chain/Mcmc proposal loop.
+ *
+ * Each entry point replaces one function of p4's CPython extension `p4.pf`
+ * (Pf/pfmodule.c, method table 2873-3031); the citation names the
+ * pfmodule.c wrapper and the Pf C function it calls.  Handles are opaque
+ * pointers (the reference passes C pointers as Python ints).  Arrays that the
+ * reference aliases from numpy (comp values, gamma val/freqs/rates,
+ * bQETneedsReset, partLikes, pre/postOrder, the var bounds) are passed here
+ * as plain host pointers and are likewise kept, not copied: the library
+ * re-reads them at every p4g_setPrams / p4g_treeLogLike / p4g_partLogLike /
+ * p4g_calculateBigPDecks call, as the reference does.
+ *
+ * Error behaviour: where the reference prints and calls exit(1), these
+ * entry points return a nonzero status (or NULL / NaN for value returns) and
+ * leave a message for p4g_lastError().  Site likelihoods <= 0 give a part
+ * log likelihood of -1.0e99, as in the reference (Pf/p4_tree.c:1134-1183).
+ *
+ * Threading: one handle is used by one host thread at a time.  All GPU work
+ * of a tree is ordered on that tree's HIP stream.
+ */
+#ifndef P4GPU_H
+#define P4GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct p4g_data p4g_data;
+typedef struct p4g_part p4g_part;
+typedef struct p4g_model p4g_model;
+typedef struct p4g_tree p4g_tree;
+typedef struct p4g_node p4g_node;
+typedef struct p4g_gdasrv p4g_gdasrv;
+typedef struct p4g_batch p4g_batch;
+
+/* ---- status ---------------------------------------------------------- */
+/* Last error message of the calling thread ("" if none).  No counterpart in
+ * the reference, which exits instead of returning errors. */
+const char *p4g_lastError(void);
+/* 1 if a usable gfx9 GPU is present and the kernels load; 0 otherwise. */
+int p4g_deviceAvailable(void);
+
+/* ---- data (Pf/part.c, Pf/data.c) -------------------------------------- */
+/* pf.newData -> newData (Pf/pfmodule.c:29-40, Pf/data.c:7) */
+p4g_data *p4g_newData(int nTax, int nParts);
+/* pf.freeData (Pf/pfmodule.c:43-55) */
+void p4g_freeData(p4g_data *d);
+/* pf.pokePartInData (Pf/pfmodule.c:73-87) */
+int p4g_pokePartInData(p4g_part *p, p4g_data *d, int partNum);
+/* pf.newPart -> newPart (Pf/pfmodule.c:112-131, Pf/part.c:15-78) */
+p4g_part *p4g_newPart(int nTax, int nChar, const char *equateSymbols, int nEquates,
+                      const char *symbols, int dim);
+/* pf.freePart (Pf/pfmodule.c:134-148) */
+void p4g_freePart(p4g_part *p);
+/* pf.pokeEquatesTable -> pokeEquatesTable (Pf/pfmodule.c:151-166, Pf/part.c:279-315):
+ * nEquates*dim characters '0'/'1', row per equate symbol. */
+int p4g_pokeEquatesTable(p4g_part *p, const char *table);
+/* pf.pokeSequences -> pokeSequences (Pf/pfmodule.c:169-183, Pf/part.c:127-275):
+ * nTax*nChar characters, taxon-major. */
+int p4g_pokeSequences(p4g_part *p, const char *seqs);
+/* pf.makePatterns -> makePatterns (Pf/pfmodule.c:186-197, Pf/part.c:317-448).
+ * Same first-occurrence order and counts; hashed instead of quadratic. */
+int p4g_makePatterns(p4g_part *p);
+/* pf.setGlobalInvarSitesVec (Pf/pfmodule.c:310-320, Pf/part.c:716-848) */
+int p4g_setGlobalInvarSitesVec(p4g_part *p);
+/* Accessors of part fields the reference reads through its struct
+ * (part->nPatterns, patternCounts, sequencePositionPatternIndex, patterns,
+ * globalInvarSitesVec/Array; Pf/pftypes.h:31-53). */
+int p4g_partNPatterns(const p4g_part *p);
+int p4g_partNChar(const p4g_part *p);
+int p4g_partDim(const p4g_part *p);
+/* Copies out: patterns [nTax][nPatterns], counts [nPatterns],
+ * seqPosPatternIndex [nChar], invarVec [nPatterns], invarArray [dim][nPatterns].
+ * Any pointer may be NULL. */
+int p4g_partGetPatterns(const p4g_part *p, int *patterns, int *counts, int *seqPosPatternIndex,
+                        int *invarVec, int *invarArray);
+/* Directly set coded patterns (codes as Pf/defines.h:33-36) instead of
+ * pokeSequences+makePatterns; used for pre-compressed input. */
+int p4g_partSetPatterns(p4g_part *p, int nPatterns, const int *patterns, const int *counts);
+/* pf.getSiteLikes (Pf/pfmodule.c:378-394): part->siteLikes after a
+ * p4g_treeLogLike(tree, getSiteLikes=1).  out: [nChar]. */
+int p4g_getSiteLikes(const p4g_part *p, double *out);
+
+/* ---- model (Pf/p4_model.c) -------------------------------------------- */
+/* pf.p4_newModel (Pf/pfmodule.c:1484-1550, Pf/p4_model.c:12-51).  The 15
+ * pointers are the var bounds arrays, aliased. */
+p4g_model *p4g_newModel(int nParts, int doRelRates, int relRatesAreFree, int nFreePrams,
+                        int isHet, int *rMatrixNormalizeTo1, double *PINVAR_MIN,
+                        double *PINVAR_MAX, double *KAPPA_MIN, double *KAPPA_MAX,
+                        double *GAMMA_SHAPE_MIN, double *GAMMA_SHAPE_MAX, double *PIVEC_MIN,
+                        double *PIVEC_MAX, double *RATE_MIN, double *RATE_MAX,
+                        double *RELRATE_MIN, double *RELRATE_MAX, double *BRLEN_MIN,
+                        double *BRLEN_MAX);
+/* pf.p4_freeModel (Pf/pfmodule.c:1554-1571) */
+void p4g_freeModel(p4g_model *m);
+/* pf.p4_newModelPart (Pf/pfmodule.c:1590-1610, Pf/p4_model.c:90-209);
+ * bQETneedsReset: aliased int32 [nComps][nRMatrices]. */
+int p4g_newModelPart(p4g_model *m, int pNum, int dim, int nComps, int nRMatrices,
+                     int nGdasrvs, int nCat, int pInvarFree, int *bQETneedsReset);
+/* pf.p4_newComp (Pf/pfmodule.c:1634-1650, Pf/p4_model.c:328-342); val aliased [dim]. */
+int p4g_newComp(p4g_model *m, int pNum, int mNum, int free, double *val);
+/* pf.p4_newRMatrix (Pf/pfmodule.c:1653-1669, Pf/p4_model.c:345-452).
+ * spec: 100 ones, 20 specified, 5 two-parameter; empirical protein specs
+ * (101-120) are not built in and are rejected. */
+int p4g_newRMatrix(p4g_model *m, int pNum, int mNum, int free, int spec);
+/* pf.p4_newGdasrv (Pf/pfmodule.c:1672-1701, Pf/p4_model.c:490-511); arrays aliased. */
+p4g_gdasrv *p4g_newGdasrv(p4g_model *m, int pNum, int mNum, int nCat, int free, double *val,
+                          double *freqs, double *rates);
+/* pf.p4_setRMatrixBigR (Pf/pfmodule.c:1720-1738): bigR[i][j] = bigR[j][i] = val */
+int p4g_setRMatrixBigR(p4g_model *m, int pNum, int mNum, int i, int j, double val);
+/* pf.p4_setKappa (Pf/pfmodule.c:1741-1757) */
+int p4g_setKappa(p4g_model *m, int pNum, int mNum, double val);
+/* pf.gdasrvCalcRates (Pf/pfmodule.c:1813-1833): DiscreteGamma into the
+ * aliased freqs/rates of the gdasrv. */
+int p4g_gdasrvCalcRates(p4g_gdasrv *g);
+/* pf.gdasrvCalcRates_np (Pf/pfmodule.c:1836-1865) */
+int p4g_gdasrvCalcRates_np(int nCat, double val, double *freqs, double *rates);
+/* pf.p4_setPInvarVal (Pf/pfmodule.c:1871-1884) */
+int p4g_setPInvarVal(p4g_model *m, int pNum, double val);
+/* pf.p4_setRelRateVal (Pf/pfmodule.c:1887-1903) */
+int p4g_setRelRateVal(p4g_model *m, int pNum, double val);
+/* pf.p4_resetBQET (Pf/pfmodule.c:1613-1631, Pf/p4_model.c:513-543): marks the
+ * (comp, rMatrix) eigensystem for recomputation at the next P build. */
+int p4g_resetBQET(p4g_model *m, int pNum, int compNum, int rMatrixNum);
+
+/* ---- tree and nodes (Pf/p4_tree.c, Pf/p4_node.c) ---------------------- */
+/* pf.p4_newTree (Pf/pfmodule.c:1383-1406, Pf/p4_tree.c:15-95).  preOrder,
+ * postOrder (int32 [nNodes], NO_ORDER = -10000 padded) and partLikes
+ * (double [nParts]) are aliased.  The reference also takes
+ * newtAndBrentPowellOptPassLimit, used only by ML optimisation. */
+p4g_tree *p4g_newTree(int nNodes, int nLeaves, int *preOrder, int *postOrder,
+                      double *partLikes, p4g_data *data, p4g_model *model);
+/* pf.p4_freeTree (Pf/pfmodule.c:1410-1421) */
+void p4g_freeTree(p4g_tree *t);
+/* pf.p4_newNode (Pf/pfmodule.c:1443-1466, Pf/p4_node.c:10-136) */
+p4g_node *p4g_newNode(int nodeNum, p4g_tree *t, int seqNum, int isLeaf, int inTree);
+/* pf.p4_freeNode (Pf/pfmodule.c:1460-1481) */
+void p4g_freeNode(p4g_node *n);
+/* pf.p4_setNodeRelation (Pf/pfmodule.c:1907-1940): relation 0 parent,
+ * 1 leftChild, 2 sibling; relNum -1 for none. */
+int p4g_setNodeRelation(p4g_node *n, int relation, int relNum);
+/* pf.p4_setTreeRoot (Pf/pfmodule.c:1944-1957) */
+int p4g_setTreeRoot(p4g_tree *t, p4g_node *n);
+/* pf.p4_setBrLen (Pf/pfmodule.c:1960-1975) */
+int p4g_setBrLen(p4g_node *n, double len);
+/* pf.p4_getBrLen / p4_getTreeLen (Pf/pfmodule.c:1978-2010) */
+double p4g_getBrLen(const p4g_node *n);
+double p4g_getTreeLen(const p4g_tree *t);
+/* pf.p4_setCompNum / p4_setRMatrixNum / p4_setGdasrvNum (Pf/pfmodule.c:2013-2058) */
+int p4g_setCompNum(p4g_node *n, int pNum, int val);
+int p4g_setRMatrixNum(p4g_node *n, int pNum, int val);
+int p4g_setGdasrvNum(p4g_node *n, int pNum, int val);
+
+/* Precision of the partials of one tree: 0 = FP64 without scaling (the
+ * reference's numerics), 1 = FP32 with per-node per-pattern power-of-two
+ * scale exponents.  No counterpart in the reference (FP64 only). */
+int p4g_setPrecision(p4g_tree *t, int fp32);
+int p4g_getPrecision(const p4g_tree *t);
+
+/* ---- evaluation ------------------------------------------------------- */
+/* pf.p4_setPrams (Pf/pfmodule.c:2086-2104, Pf/p4_tree.c:207-541):
+ * gamma rates of free gdasrvs, Q + eigensystem for every (comp, rMatrix)
+ * used, then P for every non-root node, all on the GPU.  pNum -1 = all. */
+int p4g_setPrams(p4g_tree *t, int pNum);
+/* pf.p4_treeLogLike (Pf/pfmodule.c:2132-2144, Pf/p4_tree.c:868-922). */
+int p4g_treeLogLike(p4g_tree *t, int getSiteLikes, double *lnL);
+/* pf.p4_partLogLike (Pf/pfmodule.c:2147-2160, Pf/p4_tree.c:924-1027).
+ * Recomputes the partials of nodes marked by
+ * p4g_setConditionalLikelihoodsOfInternalNodePart (and of all internal
+ * nodes on the first call), then the site sum of part pNum. */
+int p4g_partLogLike(p4g_tree *t, p4g_part *p, int pNum, int getSiteLikes, double *lnL);
+/* pf.p4_calculateBigPDecks (Pf/pfmodule.c:2165-2177, Pf/p4_node.c:288-294) */
+int p4g_calculateBigPDecks(p4g_node *n);
+/* pf.p4_calculateBigPDecksPart (Pf/p4_node.c:296-439) */
+int p4g_calculateBigPDecksPart(p4g_node *n, int pNum);
+/* pf.p4_calculateAllBigPDecksAllParts (Pf/pfmodule.c:2180-2192, Pf/p4_tree.c:815-830) */
+int p4g_calculateAllBigPDecksAllParts(p4g_tree *t);
+/* pf.p4_setConditionalLikelihoodsOfInternalNodePart (Pf/pfmodule.c:2195-2210,
+ * Pf/p4_node.c:636-857).  Deferred: marks the node; the partials of all
+ * marked nodes of the part are recomputed, in postorder, in one launch at
+ * the next p4g_partLogLike.  Results equal eager evaluation. */
+int p4g_setConditionalLikelihoodsOfInternalNodePart(p4g_node *n, int pNum);
+/* pf.p4_setConditionalLikelihoodsOfInternalNode (Pf/p4_node.c:601-634) */
+int p4g_setConditionalLikelihoodsOfInternalNode(p4g_node *n);
+
+/* ---- MCMC state sync (Pf/p4_treeCopyVerify.c) ------------------------- */
+/* pf.p4_copyCondLikes (Pf/pfmodule.c:2445-2459, Pf/p4_treeCopyVerify.c:7-31):
+ * device-to-device copy of the internal partials (and scale exponents)
+ * of tree a into tree b; doAll=0 copies only nodes with clNeedsUpdating. */
+int p4g_copyCondLikes(p4g_tree *a, p4g_tree *b, int doAll);
+/* pf.p4_copyBigPDecks (Pf/pfmodule.c:2462-2476, Pf/p4_treeCopyVerify.c:33-61) */
+int p4g_copyBigPDecks(p4g_tree *a, p4g_tree *b, int doAll);
+/* pf.p4_copyModelPrams (Pf/pfmodule.c:2479-2492, Pf/p4_treeCopyVerify.c:63-180) */
+int p4g_copyModelPrams(p4g_tree *a, p4g_tree *b);
+/* pf.p4_verifyIdentityOfTwoTrees (Pf/pfmodule.c:2431-2442,
+ * Pf/p4_treeCopyVerify.c:187-676): 0 = same (partials and P within 1e-15),
+ * 1 = different. */
+int p4g_verifyIdentityOfTwoTrees(p4g_tree *a, p4g_tree *b);
+
+/* ---- inspection (tests) ------------------------------------------------ */
+/* Copy out P of node n, part pNum: [nCat][dim][dim] FP64.  The reference
+ * exposes these through the node struct (Pf/pftypes.h:133). */
+int p4g_getBigPDeck(const p4g_node *n, int pNum, double *out);
+/* Copy out the partials of internal node n, part pNum, as
+ * [nCat][dim][nPatterns] FP64 (the reference cl layout, Pf/p4_node.c:97-106),
+ * unscaled: FP32 values are multiplied by 2^-exponent. */
+int p4g_getCondLikes(const p4g_node *n, int pNum, double *out);
+
+/* ---- batched evaluation (no single counterpart: replaces the sequential
+ * per-chain, per-part p4_partLogLike calls of Mcmc.run, p4/mcmc.py:2861-2950) */
+/* A batch evaluates the pending partials and site sums of several trees
+ * (all parts of each) in one launch.  lnL: [nTrees] (sum over parts);
+ * each tree's partLikes array is written too. */
+p4g_batch *p4g_newBatch(int nTrees, p4g_tree **trees);
+void p4g_freeBatch(p4g_batch *b);
+int p4g_batchLogLike(p4g_batch *b, double *lnL);
+/* Mean device time in microseconds of the last n launches of the pruning
+ * kernel of tree t (HIP events on the tree's stream); -1 if unknown. */
+double p4g_lastKernelMicroseconds(const p4g_tree *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1775 @@
+// p4g_api.hip -- host side of the C-ABI declared in include/p4gpu.h.
+//
+// Object model mirrors p4's Pf structs (Pf/pftypes.h:24-240): data ->
+// parts, model -> model parts -> comps / rMatrices / gdasrvs, tree ->
+// nodes.  Host-owned arrays that the reference aliases from numpy are kept
+// as pointers and re-read at every evaluation call.  Device state lives per
+// (tree, part) in TreePart (p4g_internal.h).  Every GPU operation is ordered
+// on one library stream; results come back through pinned host memory.
+#include "../../include/p4gpu.h"
+#include "p4g_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <unordered_map>
+
+struct p4g_data : p4g::Data {};
+struct p4g_part : p4g::Part {};
+struct p4g_model : p4g::Model {};
+struct p4g_tree : p4g::Tree {};
+struct p4g_node : p4g::Node {};
+struct p4g_gdasrv : p4g::Gdasrv {};
+
+namespace p4g {
+
+// ------------------------------------------------------------- utilities
+static thread_local std::string g_err;
+static bool g_timing = false;
+
+void setError(const std::string &msg) { g_err = msg; }
+void clearError() { g_err.clear(); }
+
+bool hipOk(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    setError(std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+}
+
+static hipStream_t g_stream = nullptr;
+static int g_devState = 0;  // 0 unknown, 1 ready, -1 unavailable
+
+bool deviceReady()
+{
+    if (g_devState == 1) return true;
+    if (g_devState == -1) {
+        setError("no usable AMD GPU: libp4gpu runs its likelihood kernels on the GPU only");
+        return false;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        g_devState = -1;
+        setError("no usable AMD GPU: libp4gpu runs its likelihood kernels on the GPU only");
+        return false;
+    }
+    if (!hipOk(hipSetDevice(0), "hipSetDevice")) {
+        g_devState = -1;
+        return false;
+    }
+    if (!hipOk(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate")) {
+        g_devState = -1;
+        return false;
+    }
+    g_devState = 1;
+    return true;
+}
+
+hipStream_t stream() { return g_stream; }
+
+template <typename T>
+static bool devAlloc(T **p, size_t n, const char *what)
+{
+    if (*p) return true;
+    void *q = nullptr;
+    if (!hipOk(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)), what)) return false;
+    *p = (T *)q;
+    return true;
+}
+
+template <typename T>
+static void devFree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <typename T>
+static bool hostAlloc(T **p, size_t n, const char *what)
+{
+    void *q = nullptr;
+    if (!hipOk(hipHostMalloc(&q, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault), what))
+        return false;
+    *p = (T *)q;
+    return true;
+}
+
+template <typename T>
+static void hostFree(T *&p)
+{
+    if (p) (void)hipHostFree((void *)p);
+    p = nullptr;
+}
+
+Part::~Part()
+{
+    devFree(dev.tips);
+    devFree(dev.counts);
+    devFree(dev.invMask);
+    devFree(dev.ambMask);
+}
+
+ModelPart::~ModelPart()
+{
+    for (auto *g : gdasrvs) delete static_cast<p4g_gdasrv *>(g);
+}
+
+Model::~Model()
+{
+    for (auto *mp : parts) delete mp;
+}
+
+static void freeTreePart(TreePart &t)
+{
+    devFree(t.eigIn);
+    devFree(t.eigV);
+    devFree(t.eigVi);
+    devFree(t.eigLam);
+    devFree(t.PT64);
+    devFree(t.PT32);
+    if (t.cl) (void)hipFree(t.cl);
+    t.cl = nullptr;
+    devFree(t.scl);
+    devFree(t.blockSums);
+    devFree(t.blockBad);
+    devFree(t.counter);
+    devFree(t.result);
+    devFree(t.patLikes);
+    devFree(t.prog);
+    devFree(t.rootPi);
+    devFree(t.dStage);
+    hostFree(t.hStage);
+    hostFree(t.hProg);
+    hostFree(t.hResult);
+    if (t.ev0) (void)hipEventDestroy(t.ev0);
+    if (t.ev1) (void)hipEventDestroy(t.ev1);
+    t.ev0 = t.ev1 = nullptr;
+    t.alloc = false;
+}
+
+Tree::~Tree()
+{
+    if (g_devState == 1) (void)hipStreamSynchronize(g_stream);
+    for (auto &t : tp) freeTreePart(t);
+}
+
+// ------------------------------------------------------------- data side
+static bool equateIsNLike(const Part *p, int e)
+{
+    for (int j = 0; j < p->dim; j++)
+        if (!p->equates[(size_t)e * p->dim + j]) return false;
+    return true;
+}
+
+// Pf/part.c:716-848: states compatible with every taxon, per pattern.
+static void computeInvar(Part *p)
+{
+    const int nP = p->nPatterns, dim = p->dim;
+    p->invarVec.assign(nP, 0);
+    p->invarArray.assign((size_t)dim * nP, 0);
+    for (int k = 0; k < nP; k++) {
+        int sum = 0;
+        for (int j = 0; j < dim; j++) {
+            bool ok = true;
+            for (int t = 0; t < p->nTax && ok; t++) {
+                const int code = p->patterns[(size_t)t * nP + k];
+                if (code >= 0)
+                    ok = (code == j);
+                else if (code == N_LIKE || code == GAP_CODE || code == QMARK_CODE)
+                    ok = true;
+                else
+                    ok = p->equates[(size_t)(code - EQUATES_BASE) * dim + j] != 0;
+            }
+            p->invarArray[(size_t)j * nP + k] = ok;
+            sum += ok;
+        }
+        p->invarVec[k] = sum;
+    }
+    p->invarSet = true;
+}
+
+static bool checkCodes(const Part *p)
+{
+    for (size_t i = 0; i < p->patterns.size(); i++) {
+        const int c = p->patterns[i];
+        if (c >= p->dim) {
+            setError("bad character code " + std::to_string(c) + " (>= dim)");
+            return false;
+        }
+        if (c < 0 && c != GAP_CODE && c != QMARK_CODE && c != N_LIKE &&
+            !(c >= EQUATES_BASE && c < EQUATES_BASE + p->nEquates)) {
+            setError("bad character code " + std::to_string(c));
+            return false;
+        }
+    }
+    return true;
+}
+
+// Upload one data part in the device layout of the pruning kernel.
+static bool uploadPart(Part *p)
+{
+    if (p->dev.valid) return true;
+    if (p->nPatterns < 1) {
+        setError("part has no patterns (call makePatterns first)");
+        return false;
+    }
+    if (p->dim > MAX_DIM) {
+        setError("dim " + std::to_string(p->dim) + " exceeds the supported maximum of 64");
+        return false;
+    }
+    if (!checkCodes(p)) return false;
+    if (!p->invarSet) computeInvar(p);
+    DevPart &d = p->dev;
+    const int nP = p->nPatterns;
+    const int nPatPad = (nP + 255) / 256 * 256;
+    d.nPatPad = nPatPad;
+    d.nWords = (p->dim + 31) / 32;
+    // ambiguity table: each partial (not all-state) equate gets an index
+    std::vector<int> ambIndex(p->nEquates, -1);
+    d.hostAmb.clear();
+    for (int e = 0; e < p->nEquates; e++) {
+        if (equateIsNLike(p, e)) continue;
+        uint64_t m = 0;
+        for (int j = 0; j < p->dim; j++)
+            if (p->equates[(size_t)e * p->dim + j]) m |= (1ull << j);
+        ambIndex[e] = (int)d.hostAmb.size();
+        d.hostAmb.push_back(m);
+    }
+    d.nAmb = (int)d.hostAmb.size();
+    if (p->dim + d.nAmb > (int)TIP_ALL) {
+        setError("too many ambiguity codes for the 8-bit tip alphabet");
+        return false;
+    }
+    std::vector<uint8_t> tips((size_t)p->nTax * nPatPad, (uint8_t)TIP_ALL);
+    for (int t = 0; t < p->nTax; t++)
+        for (int k = 0; k < nP; k++) {
+            const int c = p->patterns[(size_t)t * nP + k];
+            uint8_t v;
+            if (c >= 0)
+                v = (uint8_t)c;
+            else if (c == GAP_CODE || c == QMARK_CODE || c == N_LIKE)
+                v = (uint8_t)TIP_ALL;
+            else {
+                const int e = c - EQUATES_BASE;
+                v = ambIndex[e] < 0 ? (uint8_t)TIP_ALL : (uint8_t)(p->dim + ambIndex[e]);
+            }
+            tips[(size_t)t * nPatPad + k] = v;
+        }
+    std::vector<double> counts(nPatPad, 0.0);
+    for (int k = 0; k < nP; k++) counts[k] = (double)p->patternCounts[k];
+    std::vector<uint32_t> inv((size_t)d.nWords * nPatPad, 0u);
+    for (int j = 0; j < p->dim; j++)
+        for (int k = 0; k < nP; k++)
+            if (p->invarArray[(size_t)j * nP + k]) inv[(size_t)(j / 32) * nPatPad + k] |= (1u << (j % 32));
+    if (!devAlloc(&d.tips, tips.size(), "hipMalloc tips") ||
+        !devAlloc(&d.counts, counts.size(), "hipMalloc counts") ||
+        !devAlloc(&d.invMask, inv.size(), "hipMalloc invMask") ||
+        !devAlloc(&d.ambMask, std::max<size_t>(d.hostAmb.size(), 1), "hipMalloc ambMask"))
+        return false;
+    hipStream_t s = stream();
+    if (!hipOk(hipMemcpyAsync(d.tips, tips.data(), tips.size(), hipMemcpyHostToDevice, s), "copy tips") ||
+        !hipOk(hipMemcpyAsync(d.counts, counts.data(), counts.size() * 8, hipMemcpyHostToDevice, s),
+               "copy counts") ||
+        !hipOk(hipMemcpyAsync(d.invMask, inv.data(), inv.size() * 4, hipMemcpyHostToDevice, s),
+               "copy invMask"))
+        return false;
+    if (!d.hostAmb.empty() &&
+        !hipOk(hipMemcpyAsync(d.ambMask, d.hostAmb.data(), d.hostAmb.size() * 8,
+                              hipMemcpyHostToDevice, s),
+               "copy ambMask"))
+        return false;
+    if (!hipOk(hipStreamSynchronize(s), "upload part")) return false;
+    d.valid = true;
+    return true;
+}
+
+// ------------------------------------------------------------ tree side
+static int log2ceil(int K)
+{
+    int l = 0;
+    while ((1 << l) < K) l++;
+    return l;
+}
+
+static bool allocTreePart(Tree *t, int pNum)
+{
+    TreePart &tp = t->tp[pNum];
+    ModelPart *mp = t->model->parts[pNum];
+    Part *dp = t->data->parts[pNum];
+    if (!uploadPart(dp)) return false;
+    if (tp.alloc && tp.precision == t->precision) return true;
+    if (tp.alloc) {
+        // precision change: the partial buffers are re-laid out
+        if (tp.cl) (void)hipFree(tp.cl);
+        tp.cl = nullptr;
+        devFree(tp.scl);
+        tp.alloc = false;
+        std::fill(tp.dirty.begin(), tp.dirty.end(), 1);
+        tp.everComputed = false;
+    }
+    const int S = mp->dim, K = mp->nCat;
+    if (S != dp->dim) {
+        setError("model part dim differs from data part dim");
+        return false;
+    }
+    tp.dim = S;
+    tp.K = K;
+    tp.nPat = dp->nPatterns;
+    tp.nPatPad = dp->dev.nPatPad;
+    tp.precision = t->precision;
+    // slots: every non-leaf node, plus one for a leaf root
+    t->slotOf.assign(t->nNodes, -1);
+    int ns = 0;
+    for (int n = 0; n < t->nNodes; n++)
+        if (t->nodes[n] && !t->nodes[n]->isLeaf) t->slotOf[n] = ns++;
+    tp.nSlots = ns + 1;
+    tp.nEig = mp->nComps * mp->nRMatrices;
+    const size_t nPS = (size_t)t->nNodes * K * S * S;
+    const int PB = pruneBlockPatterns(K);
+    tp.maxBlocks = tp.nPatPad / PB;
+    const size_t elt = t->precision == FP32 ? sizeof(float) : sizeof(double);
+    tp.clBytes = (size_t)tp.nPatPad * K * S * elt;
+    if (!devAlloc(&tp.eigIn, (size_t)tp.nEig * (S + S * S), "hipMalloc eigIn") ||
+        !devAlloc(&tp.eigV, (size_t)tp.nEig * S * S, "hipMalloc eigV") ||
+        !devAlloc(&tp.eigVi, (size_t)tp.nEig * S * S, "hipMalloc eigVi") ||
+        !devAlloc(&tp.eigLam, (size_t)tp.nEig * S, "hipMalloc eigLam") ||
+        !devAlloc(&tp.PT64, nPS, "hipMalloc P64") || !devAlloc(&tp.PT32, nPS, "hipMalloc P32") ||
+        !devAlloc(&tp.blockSums, tp.maxBlocks, "hipMalloc blockSums") ||
+        !devAlloc(&tp.blockBad, tp.maxBlocks, "hipMalloc blockBad") ||
+        !devAlloc(&tp.counter, 1, "hipMalloc counter") || !devAlloc(&tp.result, 2, "hipMalloc result") ||
+        !devAlloc(&tp.patLikes, tp.nPatPad, "hipMalloc patLikes") ||
+        !devAlloc(&tp.rootPi, MAX_DIM, "hipMalloc rootPi"))
+        return false;
+    if (!tp.cl && !hipOk(hipMalloc(&tp.cl, tp.clBytes * tp.nSlots), "hipMalloc partials")) return false;
+    if (t->precision == FP32 && !devAlloc(&tp.scl, (size_t)tp.nSlots * tp.nPatPad, "hipMalloc scale"))
+        return false;
+    if (!tp.prog) {
+        tp.progCap = 8 * (2 * t->nNodes + 4);
+        if (!devAlloc(&tp.prog, tp.progCap, "hipMalloc program")) return false;
+        if (!hostAlloc(&tp.hProg, tp.progCap, "hipHostMalloc program")) return false;
+        tp.hProgCap = tp.progCap;
+    }
+    if (!tp.hStage) {
+        tp.hStageCap = (size_t)tp.nEig * (S + S * S) + (size_t)t->nNodes * (K + 2) + MAX_DIM + 64;
+        if (!hostAlloc(&tp.hStage, tp.hStageCap, "hipHostMalloc stage")) return false;
+        if (!devAlloc(&tp.dStage, tp.hStageCap, "hipMalloc stage")) return false;
+        if (!hostAlloc(&tp.hResult, 2, "hipHostMalloc result")) return false;
+        if (!hipOk(hipEventCreate(&tp.ev0), "hipEventCreate") ||
+            !hipOk(hipEventCreate(&tp.ev1), "hipEventCreate"))
+            return false;
+    }
+    if (!hipOk(hipMemsetAsync(tp.counter, 0, sizeof(unsigned), stream()), "zero counter")) return false;
+    tp.dirty.assign(t->nNodes, 1);
+    tp.pValid.assign(t->nNodes, 0);
+    tp.alloc = true;
+    return true;
+}
+
+static bool treeReady(Tree *t)
+{
+    if (!deviceReady()) return false;
+    if (!t->data || !t->model) {
+        setError("tree has no data or model");
+        return false;
+    }
+    if (t->root < 0 || !t->nodes[t->root]) {
+        setError("tree root is not set");
+        return false;
+    }
+    if (t->data->nParts != t->model->nParts || (int)t->tp.size() != t->model->nParts) {
+        setError("data and model part counts differ");
+        return false;
+    }
+    for (int n = 0; n < t->nNodes; n++)
+        if (!t->nodes[n]) {
+            setError("node " + std::to_string(n) + " was not created");
+            return false;
+        }
+    for (int p = 0; p < t->nParts; p++) {
+        if (!t->model->parts[p] || !t->data->parts[p]) {
+            setError("model or data part missing");
+            return false;
+        }
+        if (!allocTreePart(t, p)) return false;
+    }
+    return true;
+}
+
+// Effective branch length of Pf/p4_node.c:321-346.
+static double branchT(const Tree *t, const ModelPart *mp, const Node *nd, int pNum, int k)
+{
+    const double rate = mp->nGdasrvs ? mp->gdasrvs[nd->gdasrvNums[pNum]]->rates[k] : 1.0;
+    const double base = mp->nGdasrvs ? (nd->brLen * rate * mp->relRate) : (nd->brLen * mp->relRate);
+    (void)t;
+    return mp->pInvar == 0.0 ? base : base / (1.0 - mp->pInvar);
+}
+
+static bool checkNodeModel(const Tree *t, const ModelPart *mp, const Node *nd, int pNum)
+{
+    const int c = nd->compNums[pNum], r = nd->rMatrixNums[pNum], g = nd->gdasrvNums[pNum];
+    if (c < 0 || c >= mp->nComps || (nd->nodeNum != t->root && (r < 0 || r >= mp->nRMatrices)) ||
+        (mp->nGdasrvs && nd->nodeNum != t->root && (g < 0 || g >= mp->nGdasrvs))) {
+        setError("node " + std::to_string(nd->nodeNum) + " has an out-of-range model component");
+        return false;
+    }
+    if (mp->comps[c].val == nullptr) {
+        setError("comp " + std::to_string(c) + " has no values");
+        return false;
+    }
+    return true;
+}
+
+// Compute the eigensystems of the listed (comp, rMatrix) slots on the GPU.
+static bool runEig(Tree *t, int pNum, const std::vector<int> &slots)
+{
+    if (slots.empty()) return true;
+    TreePart &tp = t->tp[pNum];
+    ModelPart *mp = t->model->parts[pNum];
+    const int S = mp->dim;
+    double *h = tp.hStage;
+    int *hs = reinterpret_cast<int *>(h + (size_t)tp.nEig * (S + S * S));
+    // Reuse the staging area: inputs for all slots, then the slot list.
+    for (int slot : slots) {
+        const int c = slot / mp->nRMatrices, r = slot % mp->nRMatrices;
+        double *dst = h + (size_t)slot * (S + S * S);
+        std::memcpy(dst, mp->comps[c].val, sizeof(double) * S);
+        std::memcpy(dst + S, mp->rMatrices[r].bigR.data(), sizeof(double) * S * S);
+    }
+    for (size_t i = 0; i < slots.size(); i++) hs[i] = slots[i];
+    hipStream_t s = stream();
+    const size_t inWords = (size_t)tp.nEig * (S + S * S);
+    const size_t bytes = sizeof(double) * inWords + sizeof(int) * slots.size();
+    if (!hipOk(hipMemcpyAsync(tp.dStage, h, bytes, hipMemcpyHostToDevice, s), "stage eigen inputs"))
+        return false;
+    const int *dSlots = reinterpret_cast<const int *>(tp.dStage + inWords);
+    EigArgs a{S, (int)slots.size(), dSlots, tp.dStage, tp.eigV, tp.eigVi, tp.eigLam};
+    if (!hipOk(launchEig(a, s), "launch eigensystem kernel")) return false;
+    // the staging buffers are reused by the next host step: wait here
+    return hipOk(hipStreamSynchronize(s), "eigensystem kernel");
+}
+
+static bool runPmat(Tree *t, int pNum, const std::vector<int> &nodes)
+{
+    if (nodes.empty()) return true;
+    TreePart &tp = t->tp[pNum];
+    ModelPart *mp = t->model->parts[pNum];
+    const int K = mp->nCat;
+    const int nI = (int)nodes.size();
+    double *ht = tp.hStage;
+    int *hn = reinterpret_cast<int *>(ht + (size_t)nI * K);
+    int *he = hn + nI;
+    for (int i = 0; i < nI; i++) {
+        const Node *nd = t->nodes[nodes[i]];
+        for (int k = 0; k < K; k++) ht[(size_t)i * K + k] = branchT(t, mp, nd, pNum, k);
+        hn[i] = nodes[i];
+        he[i] = nd->compNums[pNum] * mp->nRMatrices + nd->rMatrixNums[pNum];
+    }
+    const size_t bytes = sizeof(double) * nI * K + sizeof(int) * 2 * nI;
+    hipStream_t s = stream();
+    if (bytes > tp.hStageCap * sizeof(double)) {
+        setError("internal: staging buffer too small");
+        return false;
+    }
+    char *d = reinterpret_cast<char *>(tp.dStage);
+    if (!hipOk(hipMemcpyAsync(d, ht, bytes, hipMemcpyHostToDevice, s), "stage P inputs")) return false;
+    PmatArgs a{mp->dim, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
+               reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
+               reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
+    if (!hipOk(launchPmat(a, s), "launch P kernel")) return false;
+    if (!hipOk(hipStreamSynchronize(s), "P kernel")) return false;
+    for (int n : nodes) tp.pValid[n] = 1;
+    return true;
+}
+
+// Pf/p4_tree.c:218-541 for one part.
+static bool setPramsPart(Tree *t, int pNum)
+{
+    ModelPart *mp = t->model->parts[pNum];
+    const int S = mp->dim;
+    if (mp->nCat > 1)
+        for (auto *g : mp->gdasrvs)
+            if (g->free) discreteGamma(g->freqs, g->rates, g->val[0], g->val[0], mp->nCat, 0);
+    for (auto &r : mp->rMatrices)
+        if (r.free && r.spec == RMATRIX_2P && S == 4) {
+            const double al = 1.0 / 3.0, be = al * r.kappa;
+            const double R[16] = {0, al, be, al, al, 0, al, be, be, al, 0, al, al, be, al, 0};
+            r.bigR.assign(R, R + 16);
+        }
+    const double pivecMin = t->model->PIVEC_MIN ? t->model->PIVEC_MIN[0] : 1e-13;
+    for (int c = 0; c < mp->nComps; c++) {
+        if (!mp->comps[c].val) {
+            setError("comp " + std::to_string(c) + " of part " + std::to_string(pNum) + " has no values");
+            return false;
+        }
+        double sum = 0.0;
+        for (int j = 0; j < S; j++) {
+            if (mp->comps[c].val[j] < pivecMin) {
+                char b[160];
+                snprintf(b, sizeof b, "p4_setPramsPart() part %d, comp %d, value %d is %g   Bad.", pNum, c, j,
+                         mp->comps[c].val[j]);
+                setError(b);
+                return false;
+            }
+            sum += mp->comps[c].val[j];
+        }
+        if (std::fabs(sum - 1.0) > 1e-14) {
+            char b[160];
+            snprintf(b, sizeof b, "p4_setPramsPart() part %d, comp %d, values do not sum to 1.0.  sum=%.17g",
+                     pNum, c, sum);
+            setError(b);
+            return false;
+        }
+    }
+    if (mp->bQETneedsReset)
+        for (int i = 0; i < mp->nComps * mp->nRMatrices; i++) mp->bQETneedsReset[i] = 1;
+    std::vector<char> used(mp->nComps * mp->nRMatrices, 0);
+    std::vector<int> slots, nodes;
+    for (int n = 0; n < t->nNodes; n++) {
+        const Node *nd = t->nodes[n];
+        if (n == t->root) continue;
+        if (!checkNodeModel(t, mp, nd, pNum)) return false;
+        const int slot = nd->compNums[pNum] * mp->nRMatrices + nd->rMatrixNums[pNum];
+        if (!used[slot]) {
+            used[slot] = 1;
+            slots.push_back(slot);
+        }
+        nodes.push_back(n);
+    }
+    if (!runEig(t, pNum, slots)) return false;
+    if (mp->bQETneedsReset)
+        for (int slot : slots) mp->bQETneedsReset[slot] = 0;
+    return runPmat(t, pNum, nodes);
+}
+
+// P of one node (Pf/p4_node.c:296-346), recomputing a stale eigensystem
+// first as the reference does (Pf/p4_node.c:310-315).
+static bool bigPDecksPart(Tree *t, Node *nd, int pNum)
+{
+    if (nd->nodeNum == t->root) return true;
+    ModelPart *mp = t->model->parts[pNum];
+    if (!checkNodeModel(t, mp, nd, pNum)) return false;
+    const int slot = nd->compNums[pNum] * mp->nRMatrices + nd->rMatrixNums[pNum];
+    if (mp->bQETneedsReset && mp->bQETneedsReset[slot]) {
+        if (!runEig(t, pNum, {slot})) return false;
+        mp->bQETneedsReset[slot] = 0;
+    }
+    return runPmat(t, pNum, {nd->nodeNum});
+}
+
+// Build the node program for part pNum: all internal nodes in postorder
+// (full) or only the marked ones, always ending with the root.
+static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsOut)
+{
+    TreePart &tp = t->tp[pNum];
+    std::vector<OpRec> ops;
+    std::vector<KidRec> kids;
+    std::vector<char> have(t->nNodes, 0);
+    for (int n = 0; n < t->nNodes; n++) have[n] = (!t->nodes[n]->isLeaf && tp.everComputed && !tp.dirty[n]);
+    for (int j = 0; j < t->nNodes; j++) {
+        const int n = t->postOrder[j];
+        if (n == NO_ORDER) continue;
+        if (n < 0 || n >= t->nNodes) {
+            setError("postOrder holds an out-of-range node number");
+            return false;
+        }
+        const Node *nd = t->nodes[n];
+        const bool isRoot = (n == t->root);
+        if (nd->isLeaf && !isRoot) continue;
+        if (!isRoot && !full && tp.everComputed && !tp.dirty[n]) continue;
+        OpRec op{isRoot ? -1 : t->slotOf[n], isRoot ? OP_ROOT : 0, 0, (int)kids.size()};
+        if (isRoot && nd->isLeaf) op.dst = tp.nSlots - 1;
+        int c = nd->leftChild;
+        while (c >= 0) {
+            if (c >= t->nNodes) {
+                setError("node relation out of range");
+                return false;
+            }
+            const Node *cn = t->nodes[c];
+            KidRec k{};
+            if (cn->isLeaf) {
+                if (cn->seqNum < 0 || cn->seqNum >= t->data->nTax) {
+                    setError("leaf node " + std::to_string(c) + " has no sequence");
+                    return false;
+                }
+                k.kind = 0;
+                k.idx = cn->seqNum;
+            } else {
+                if (!have[c]) {
+                    setError("partials of node " + std::to_string(c) +
+                             " are needed but were never computed (postorder or marks are incomplete)");
+                    return false;
+                }
+                k.kind = 1;
+                k.idx = t->slotOf[c];
+            }
+            if (!tp.pValid[c]) {
+                setError("P matrices of node " + std::to_string(c) + " were never computed (call setPrams)");
+                return false;
+            }
+            k.pnode = c;
+            kids.push_back(k);
+            op.nKids++;
+            if (op.nKids > MAX_CHILDREN) {
+                setError("too many children");
+                return false;
+            }
+            c = cn->sibling;
+        }
+        if (op.nKids == 0) {
+            setError("internal node " + std::to_string(n) + " has no children");
+            return false;
+        }
+        if (!isRoot) have[n] = 1;
+        ops.push_back(op);
+        if (isRoot) break;
+    }
+    if (ops.empty() || !(ops.back().flags & OP_ROOT)) {
+        setError("postOrder does not end at the root");
+        return false;
+    }
+    const size_t words = ops.size() * 4 + kids.size() * 4;
+    if ((int)words > tp.hProgCap) {
+        hostFree(tp.hProg);
+        devFree(tp.prog);
+        tp.progCap = tp.hProgCap = (int)words * 2;
+        if (!devAlloc(&tp.prog, tp.progCap, "hipMalloc program") ||
+            !hostAlloc(&tp.hProg, tp.hProgCap, "hipHostMalloc program"))
+            return false;
+    }
+    std::memcpy(tp.hProg, ops.data(), ops.size() * sizeof(OpRec));
+    std::memcpy(tp.hProg + ops.size() * 4, kids.data(), kids.size() * sizeof(KidRec));
+    *nOpsOut = (int)ops.size();
+    *nKidsOut = (int)kids.size();
+    return true;
+}
+
+static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLikes)
+{
+    TreePart &tp = t->tp[pNum];
+    ModelPart *mp = t->model->parts[pNum];
+    Part *dp = t->data->parts[pNum];
+    const Node *root = t->nodes[t->root];
+    a.ops = reinterpret_cast<const OpRec *>(tp.prog);
+    a.kids = reinterpret_cast<const KidRec *>(tp.prog + nOps * 4);
+    a.nOps = nOps;
+    a.dim = mp->dim;
+    a.K = mp->nCat;
+    a.log2Kp = log2ceil(mp->nCat);
+    a.nPat = tp.nPat;
+    a.nPatPad = tp.nPatPad;
+    a.tips = dp->dev.tips;
+    a.PT = t->precision == FP32 ? (const void *)tp.PT32 : (const void *)tp.PT64;
+    a.cl = tp.cl;
+    a.scl = tp.scl;
+    a.counts = dp->dev.counts;
+    a.invMask = dp->dev.invMask;
+    a.nWords = dp->dev.nWords;
+    a.ambMask = dp->dev.ambMask;
+    a.rootPi = tp.rootPi;
+    a.pInvar = mp->pInvar;
+    a.rootLeafRow = root->isLeaf ? root->seqNum : -1;
+    a.blockSums = tp.blockSums;
+    a.blockBad = tp.blockBad;
+    a.counter = tp.counter;
+    a.result = tp.result;
+    a.patLikes = siteLikes ? tp.patLikes : nullptr;
+}
+
+// Stage the program and root frequencies of one part (async).
+static bool stagePart(Tree *t, int pNum, bool full, int *nOps)
+{
+    TreePart &tp = t->tp[pNum];
+    ModelPart *mp = t->model->parts[pNum];
+    const Node *root = t->nodes[t->root];
+    int nKids = 0;
+    if (!buildProgram(t, pNum, full, nOps, &nKids)) return false;
+    const int rc = root->compNums[pNum];
+    if (rc < 0 || rc >= mp->nComps || !mp->comps[rc].val) {
+        setError("root comp out of range");
+        return false;
+    }
+    double *hpi = tp.hStage + (tp.hStageCap - MAX_DIM);
+    std::memcpy(hpi, mp->comps[rc].val, sizeof(double) * mp->dim);
+    hipStream_t s = stream();
+    return hipOk(hipMemcpyAsync(tp.prog, tp.hProg, sizeof(int) * 4 * (*nOps + nKids),
+                                hipMemcpyHostToDevice, s),
+                 "stage program") &&
+           hipOk(hipMemcpyAsync(tp.rootPi, hpi, sizeof(double) * mp->dim, hipMemcpyHostToDevice, s),
+                 "stage root comp");
+}
+
+static void finishPart(Tree *t, int pNum, bool siteLikes, double lnL, bool bad)
+{
+    TreePart &tp = t->tp[pNum];
+    std::fill(tp.dirty.begin(), tp.dirty.end(), 0);
+    tp.everComputed = true;
+    if (t->partLikes) t->partLikes[pNum] = lnL;
+    (void)siteLikes;
+    (void)bad;
+}
+
+// Evaluate pending partials + site sum of one part (one launch).
+static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
+{
+    TreePart &tp = t->tp[pNum];
+    Part *dp = t->data->parts[pNum];
+    int nOps = 0;
+    if (!stagePart(t, pNum, full, &nOps)) return false;
+    PruneArgs a{};
+    fillPruneArgs(t, pNum, nOps, a, siteLikes);
+    hipStream_t s = stream();
+    const int nBlocks = tp.nPatPad / pruneBlockPatterns(a.K);
+    if (g_timing) (void)hipEventRecord(tp.ev0, s);
+    if (!hipOk(launchPrune(a, t->precision, nBlocks, s), "launch pruning kernel")) return false;
+    if (g_timing) (void)hipEventRecord(tp.ev1, s);
+    if (!hipOk(hipMemcpyAsync(tp.hResult, tp.result, 2 * sizeof(double), hipMemcpyDeviceToHost, s),
+               "copy result"))
+        return false;
+    std::vector<double> pl;
+    if (siteLikes) {
+        pl.resize(tp.nPat);
+        if (!hipOk(hipMemcpyAsync(pl.data(), tp.patLikes, sizeof(double) * tp.nPat,
+                                  hipMemcpyDeviceToHost, s),
+                   "copy site likes"))
+            return false;
+    }
+    if (!hipOk(hipStreamSynchronize(s), "pruning kernel")) return false;
+    if (g_timing) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, tp.ev0, tp.ev1) == hipSuccess) tp.lastKernelUs = ms * 1000.0;
+    }
+    const bool bad = tp.hResult[1] != 0.0;
+    *lnL = tp.hResult[0];
+    if (siteLikes && !bad) {
+        dp->siteLikes.assign(dp->nChar, 0.0);
+        if ((int)dp->seqPosPatternIndex.size() == dp->nChar)
+            for (int i = 0; i < dp->nChar; i++) dp->siteLikes[i] = pl[dp->seqPosPatternIndex[i]];
+        else {
+            dp->siteLikes.assign(pl.begin(), pl.end());
+        }
+    }
+    finishPart(t, pNum, siteLikes, *lnL, bad);
+    return true;
+}
+
+}  // namespace p4g
+
+using namespace p4g;
+
+// ===================================================================== C-ABI
+extern "C" {
+
+const char *p4g_lastError(void) { return g_err.c_str(); }
+
+int p4g_deviceAvailable(void) { return deviceReady() ? 1 : 0; }
+
+void p4g_setKernelTiming(int on) { g_timing = on != 0; }
+
+double p4g_lastKernelMicroseconds(const p4g_tree *t)
+{
+    if (!t || t->tp.empty()) return -1.0;
+    double s = 0.0;
+    int n = 0;
+    for (auto &tp : t->tp)
+        if (tp.lastKernelUs >= 0) {
+            s += tp.lastKernelUs;
+            n++;
+        }
+    return n ? s : -1.0;
+}
+
+// ---------------------------------------------------------------- data
+p4g_data *p4g_newData(int nTax, int nParts)
+{
+    clearError();
+    if (nTax < 1 || nParts < 1) {
+        setError("newData: nTax and nParts must be positive");
+        return nullptr;
+    }
+    auto *d = new p4g_data();
+    d->nTax = nTax;
+    d->nParts = nParts;
+    d->parts.assign(nParts, nullptr);
+    return d;
+}
+
+void p4g_freeData(p4g_data *d) { delete d; }
+
+int p4g_pokePartInData(p4g_part *p, p4g_data *d, int partNum)
+{
+    clearError();
+    if (!p || !d || partNum < 0 || partNum >= d->nParts) {
+        setError("pokePartInData: bad arguments");
+        return 1;
+    }
+    d->parts[partNum] = p;
+    p->data = d;
+    return 0;
+}
+
+p4g_part *p4g_newPart(int nTax, int nChar, const char *equateSymbols, int nEquates,
+                      const char *symbols, int dim)
+{
+    clearError();
+    if (nTax < 1 || nChar < 1 || dim < 2 || nEquates < 0) {
+        setError("newPart: bad sizes");
+        return nullptr;
+    }
+    auto *p = new p4g_part();
+    p->nTax = nTax;
+    p->nChar = nChar;
+    p->dim = dim;
+    p->nEquates = nEquates;
+    p->symbols = symbols ? std::string(symbols, strnlen(symbols, dim)) : std::string();
+    p->equateSymbols = equateSymbols ? std::string(equateSymbols, strnlen(equateSymbols, nEquates)) : std::string();
+    p->equates.assign((size_t)nEquates * dim, 0);
+    p->sequences.assign((size_t)nTax * nChar, 0);
+    return p;
+}
+
+void p4g_freePart(p4g_part *p) { delete p; }
+
+int p4g_pokeEquatesTable(p4g_part *p, const char *table)
+{
+    clearError();
+    if (!p || !table) return 1;
+    for (int i = 0; i < p->nEquates * p->dim; i++) p->equates[i] = (table[i] == '1') ? 1 : 0;
+    p->dev.valid = false;
+    return 0;
+}
+
+// Pf/part.c:127-275: symbols -> 0..dim-1, '-' gap, '?' qmark, equates.
+int p4g_pokeSequences(p4g_part *p, const char *seqs)
+{
+    clearError();
+    if (!p || !seqs) return 1;
+    int lut[256];
+    for (int i = 0; i < 256; i++) lut[i] = INT32_MIN;
+    for (int e = p->nEquates - 1; e >= 0; e--) lut[(unsigned char)p->equateSymbols[e]] = EQUATES_BASE + e;
+    lut[(unsigned char)'-'] = GAP_CODE;
+    lut[(unsigned char)'?'] = QMARK_CODE;
+    for (int m = p->dim - 1; m >= 0; m--)
+        if (m < (int)p->symbols.size()) lut[(unsigned char)p->symbols[m]] = m;
+    const size_t n = (size_t)p->nTax * p->nChar;
+    for (size_t i = 0; i < n; i++) {
+        const int c = lut[(unsigned char)seqs[i]];
+        if (c == INT32_MIN) {
+            char b[128];
+            snprintf(b, sizeof b, "pokeSequences: got character '%c', neither a symbol nor an equate", seqs[i]);
+            setError(b);
+            return 1;
+        }
+        p->sequences[i] = c;
+    }
+    p->dev.valid = false;
+    return 0;
+}
+
+// Pf/part.c:317-448: patterns in first-occurrence order with counts.  A hash
+// of each column replaces the reference's quadratic search.
+int p4g_makePatterns(p4g_part *p)
+{
+    clearError();
+    if (!p) return 1;
+    const int nTax = p->nTax, nChar = p->nChar;
+    std::unordered_map<std::string, int> idx;
+    idx.reserve((size_t)nChar * 2);
+    std::string key((size_t)nTax * 2, '\0');
+    std::vector<int> firstSite;
+    p->patternCounts.clear();
+    p->seqPosPatternIndex.assign(nChar, 0);
+    for (int i = 0; i < nChar; i++) {
+        for (int t = 0; t < nTax; t++) {
+            const int c = p->sequences[(size_t)t * nChar + i] + 128;
+            key[2 * t] = (char)(c & 0xff);
+            key[2 * t + 1] = (char)((c >> 8) & 0xff);
+        }
+        auto it = idx.find(key);
+        if (it != idx.end()) {
+            p->patternCounts[it->second]++;
+            p->seqPosPatternIndex[i] = it->second;
+        } else {
+            const int k = (int)firstSite.size();
+            idx.emplace(key, k);
+            firstSite.push_back(i);
+            p->patternCounts.push_back(1);
+            p->seqPosPatternIndex[i] = k;
+        }
+    }
+    const int nP = (int)firstSite.size();
+    p->nPatterns = nP;
+    p->patterns.assign((size_t)nTax * nP, 0);
+    for (int t = 0; t < nTax; t++)
+        for (int k = 0; k < nP; k++) p->patterns[(size_t)t * nP + k] = p->sequences[(size_t)t * nChar + firstSite[k]];
+    p->invarSet = false;
+    p->dev.valid = false;
+    if (!checkCodes(p)) return 1;
+    return 0;
+}
+
+int p4g_setGlobalInvarSitesVec(p4g_part *p)
+{
+    clearError();
+    if (!p || p->nPatterns < 1) {
+        setError("setGlobalInvarSitesVec: no patterns");
+        return 1;
+    }
+    computeInvar(p);
+    p->dev.valid = false;
+    return 0;
+}
+
+int p4g_partNPatterns(const p4g_part *p) { return p ? p->nPatterns : -1; }
+int p4g_partNChar(const p4g_part *p) { return p ? p->nChar : -1; }
+int p4g_partDim(const p4g_part *p) { return p ? p->dim : -1; }
+
+int p4g_partGetPatterns(const p4g_part *p, int *patterns, int *counts, int *seqPosPatternIndex,
+                        int *invarVec, int *invarArray)
+{
+    if (!p) return 1;
+    if (patterns) std::copy(p->patterns.begin(), p->patterns.end(), patterns);
+    if (counts) std::copy(p->patternCounts.begin(), p->patternCounts.end(), counts);
+    if (seqPosPatternIndex) std::copy(p->seqPosPatternIndex.begin(), p->seqPosPatternIndex.end(), seqPosPatternIndex);
+    if (invarVec || invarArray) {
+        if (!p->invarSet) computeInvar(const_cast<p4g_part *>(p));
+        if (invarVec) std::copy(p->invarVec.begin(), p->invarVec.end(), invarVec);
+        if (invarArray) std::copy(p->invarArray.begin(), p->invarArray.end(), invarArray);
+    }
+    return 0;
+}
+
+int p4g_partSetPatterns(p4g_part *p, int nPatterns, const int *patterns, const int *counts)
+{
+    clearError();
+    if (!p || nPatterns < 1 || !patterns || !counts) {
+        setError("partSetPatterns: bad arguments");
+        return 1;
+    }
+    p->nPatterns = nPatterns;
+    p->patterns.assign(patterns, patterns + (size_t)p->nTax * nPatterns);
+    p->patternCounts.assign(counts, counts + nPatterns);
+    p->seqPosPatternIndex.clear();
+    int n = 0;
+    for (int k = 0; k < nPatterns; k++) n += counts[k];
+    p->nChar = n;
+    p->invarSet = false;
+    p->dev.valid = false;
+    return checkCodes(p) ? 0 : 1;
+}
+
+int p4g_getSiteLikes(const p4g_part *p, double *out)
+{
+    if (!p || p->siteLikes.empty()) {
+        setError("getSiteLikes: no site likelihoods (evaluate with getSiteLikes=1 first)");
+        return 1;
+    }
+    std::copy(p->siteLikes.begin(), p->siteLikes.end(), out);
+    return 0;
+}
+
+// --------------------------------------------------------------- model
+p4g_model *p4g_newModel(int nParts, int doRelRates, int relRatesAreFree, int nFreePrams, int isHet,
+                        int *rMatrixNormalizeTo1, double *PINVAR_MIN, double *PINVAR_MAX,
+                        double *KAPPA_MIN, double *KAPPA_MAX, double *GAMMA_SHAPE_MIN,
+                        double *GAMMA_SHAPE_MAX, double *PIVEC_MIN, double *PIVEC_MAX,
+                        double *RATE_MIN, double *RATE_MAX, double *RELRATE_MIN,
+                        double *RELRATE_MAX, double *BRLEN_MIN, double *BRLEN_MAX)
+{
+    clearError();
+    if (nParts < 1) {
+        setError("newModel: nParts must be positive");
+        return nullptr;
+    }
+    auto *m = new p4g_model();
+    m->nParts = nParts;
+    m->doRelRates = doRelRates;
+    m->relRatesAreFree = relRatesAreFree;
+    m->nFreePrams = nFreePrams;
+    m->isHet = isHet;
+    m->rMatrixNormalizeTo1 = rMatrixNormalizeTo1;
+    m->PINVAR_MIN = PINVAR_MIN;
+    m->PINVAR_MAX = PINVAR_MAX;
+    m->KAPPA_MIN = KAPPA_MIN;
+    m->KAPPA_MAX = KAPPA_MAX;
+    m->GAMMA_SHAPE_MIN = GAMMA_SHAPE_MIN;
+    m->GAMMA_SHAPE_MAX = GAMMA_SHAPE_MAX;
+    m->PIVEC_MIN = PIVEC_MIN;
+    m->PIVEC_MAX = PIVEC_MAX;
+    m->RATE_MIN = RATE_MIN;
+    m->RATE_MAX = RATE_MAX;
+    m->RELRATE_MIN = RELRATE_MIN;
+    m->RELRATE_MAX = RELRATE_MAX;
+    m->BRLEN_MIN = BRLEN_MIN;
+    m->BRLEN_MAX = BRLEN_MAX;
+    m->parts.assign(nParts, nullptr);
+    return m;
+}
+
+void p4g_freeModel(p4g_model *m) { delete m; }
+
+static ModelPart *modelPart(p4g_model *m, int pNum)
+{
+    if (!m || pNum < 0 || pNum >= m->nParts || !m->parts[pNum]) {
+        setError("model part " + std::to_string(pNum) + " does not exist");
+        return nullptr;
+    }
+    return m->parts[pNum];
+}
+
+int p4g_newModelPart(p4g_model *m, int pNum, int dim, int nComps, int nRMatrices, int nGdasrvs,
+                     int nCat, int pInvarFree, int *bQETneedsReset)
+{
+    clearError();
+    if (!m || pNum < 0 || pNum >= m->nParts || dim < 2 || dim > MAX_DIM || nComps < 1 ||
+        nRMatrices < 1 || nGdasrvs < 0 || nCat < 1 || (nCat > 1 && nGdasrvs < 1)) {
+        setError("newModelPart: bad arguments");
+        return 1;
+    }
+    auto *mp = new ModelPart();
+    mp->dim = dim;
+    mp->nComps = nComps;
+    mp->nRMatrices = nRMatrices;
+    mp->nGdasrvs = nGdasrvs;
+    mp->nCat = nCat;
+    mp->pInvarFree = pInvarFree;
+    mp->comps.resize(nComps);
+    mp->rMatrices.resize(nRMatrices);
+    mp->gdasrvs.assign(nGdasrvs, nullptr);
+    mp->pInvar = -1.0;
+    mp->relRate = -1.0;
+    mp->bQETneedsReset = bQETneedsReset;
+    delete m->parts[pNum];
+    m->parts[pNum] = mp;
+    return 0;
+}
+
+int p4g_newComp(p4g_model *m, int pNum, int mNum, int free, double *val)
+{
+    clearError();
+    ModelPart *mp = modelPart(m, pNum);
+    if (!mp || mNum < 0 || mNum >= mp->nComps || !val) {
+        if (mp) setError("newComp: bad arguments");
+        return 1;
+    }
+    mp->comps[mNum].free = free;
+    mp->comps[mNum].val = val;
+    return 0;
+}
+
+int p4g_newRMatrix(p4g_model *m, int pNum, int mNum, int free, int spec)
+{
+    clearError();
+    ModelPart *mp = modelPart(m, pNum);
+    if (!mp || mNum < 0 || mNum >= mp->nRMatrices) {
+        if (mp) setError("newRMatrix: bad arguments");
+        return 1;
+    }
+    if (spec > RMATRIX_POISSON) {
+        setError("newRMatrix: empirical protein rate matrices (spec " + std::to_string(spec) +
+                 ") are not built in; give the 190 exchangeabilities as a 'specified' rMatrix");
+        return 1;
+    }
+    if (spec == RMATRIX_2P && mp->dim != 4) {
+        setError("newRMatrix: the 2p matrix is for 4-state data");
+        return 1;
+    }
+    RMatrix &r = mp->rMatrices[mNum];
+    r.free = free;
+    r.spec = spec;
+    r.kappa = 2.0;
+    r.bigR.assign((size_t)mp->dim * mp->dim, 1.0);
+    if (spec == RMATRIX_2P) {
+        const double al = 1.0 / 3.0, be = al * r.kappa;
+        const double R[16] = {0, al, be, al, al, 0, al, be, be, al, 0, al, al, be, al, 0};
+        r.bigR.assign(R, R + 16);
+    }
+    return 0;
+}
+
+p4g_gdasrv *p4g_newGdasrv(p4g_model *m, int pNum, int mNum, int nCat, int free, double *val,
+                          double *freqs, double *rates)
+{
+    clearError();
+    ModelPart *mp = modelPart(m, pNum);
+    if (!mp || mNum < 0 || mNum >= mp->nGdasrvs || !val || !freqs || !rates || nCat != mp->nCat) {
+        if (mp) setError("newGdasrv: bad arguments");
+        return nullptr;
+    }
+    auto *g = new p4g_gdasrv();
+    g->free = free;
+    g->nCat = nCat;
+    g->val = val;
+    g->freqs = freqs;
+    g->rates = rates;
+    delete static_cast<p4g_gdasrv *>(mp->gdasrvs[mNum]);
+    mp->gdasrvs[mNum] = g;
+    return g;
+}
+
+int p4g_setRMatrixBigR(p4g_model *m, int pNum, int mNum, int i, int j, double val)
+{
+    clearError();
+    ModelPart *mp = modelPart(m, pNum);
+    if (!mp || mNum < 0 || mNum >= mp->nRMatrices || i < 0 || j < 0 || i >= mp->dim || j >= mp->dim) {
+        if (mp) setError("setRMatrixBigR: bad arguments");
+        return 1;
+    }
+    mp->rMatrices[mNum].bigR[(size_t)i * mp->dim + j] = val;
+    mp->rMatrices[mNum].bigR[(size_t)j * mp->dim + i] = val;
+    return 0;
+}
+
+int p4g_setKappa(p4g_model *m, int pNum, int mNum, double val)
+{
+    clearError();
+    ModelPart *mp = modelPart(m, pNum);
+    if (!mp || mNum < 0 || mNum >= mp->nRMatrices) return 1;
+    mp->rMatrices[mNum].kappa = val;
+    return 0;
+}
+
+int p4g_gdasrvCalcRates(p4g_gdasrv *g)
+{
+    clearError();
+    if (!g) return 1;
+    return discreteGamma(g->freqs, g->rates, g->val[0], g->val[0], g->nCat, 0);
+}
+
+int p4g_gdasrvCalcRates_np(int nCat, double val, double *freqs, double *rates)
+{
+    clearError();
+    if (nCat < 1 || !freqs || !rates) return 1;
+    return discreteGamma(freqs, rates, val, val, nCat, 0);
+}
+
+int p4g_setPInvarVal(p4g_model *m, int pNum, double val)
+{
+    clearError();
+    ModelPart *mp = modelPart(m, pNum);
+    if (!mp) return 1;
+    if (!(val >= 0.0 && val < 1.0)) {
+        setError("setPInvarVal: pInvar must be in [0, 1)");
+        return 1;
+    }
+    mp->pInvar = val;
+    return 0;
+}
+
+int p4g_setRelRateVal(p4g_model *m, int pNum, double val)
+{
+    clearError();
+    ModelPart *mp = modelPart(m, pNum);
+    if (!mp) return 1;
+    mp->relRate = val;
+    return 0;
+}
+
+int p4g_resetBQET(p4g_model *m, int pNum, int compNum, int rMatrixNum)
+{
+    clearError();
+    ModelPart *mp = modelPart(m, pNum);
+    if (!mp || compNum < 0 || compNum >= mp->nComps || rMatrixNum < 0 || rMatrixNum >= mp->nRMatrices)
+        return 1;
+    if (mp->bQETneedsReset) mp->bQETneedsReset[compNum * mp->nRMatrices + rMatrixNum] = 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------- tree
+p4g_tree *p4g_newTree(int nNodes, int nLeaves, int *preOrder, int *postOrder, double *partLikes,
+                      p4g_data *data, p4g_model *model)
+{
+    clearError();
+    if (nNodes < 1 || !preOrder || !postOrder || !data || !model) {
+        setError("newTree: bad arguments");
+        return nullptr;
+    }
+    if (data->nParts != model->nParts) {
+        setError("newTree: data and model part counts differ");
+        return nullptr;
+    }
+    auto *t = new p4g_tree();
+    t->nNodes = nNodes;
+    t->nLeaves = nLeaves;
+    t->nodes.assign(nNodes, nullptr);
+    t->preOrder = preOrder;
+    t->postOrder = postOrder;
+    t->partLikes = partLikes;
+    t->data = data;
+    t->model = model;
+    t->nParts = model->nParts;
+    t->tp.resize(t->nParts);
+    return t;
+}
+
+void p4g_freeTree(p4g_tree *t)
+{
+    if (!t) return;
+    for (auto *n : t->nodes) delete static_cast<p4g_node *>(n);
+    delete t;
+}
+
+p4g_node *p4g_newNode(int nodeNum, p4g_tree *t, int seqNum, int isLeaf, int inTree)
+{
+    clearError();
+    if (!t || nodeNum < 0 || nodeNum >= t->nNodes) {
+        setError("newNode: bad node number");
+        return nullptr;
+    }
+    auto *n = new p4g_node();
+    n->nodeNum = nodeNum;
+    n->tree = t;
+    n->seqNum = seqNum;
+    n->isLeaf = isLeaf;
+    n->inTree = inTree;
+    n->compNums.assign(t->nParts, 0);
+    n->rMatrixNums.assign(t->nParts, 0);
+    n->gdasrvNums.assign(t->nParts, 0);
+    delete static_cast<p4g_node *>(t->nodes[nodeNum]);
+    t->nodes[nodeNum] = n;
+    for (auto &tp : t->tp) {
+        if (tp.alloc) freeTreePart(tp);
+    }
+    return n;
+}
+
+void p4g_freeNode(p4g_node *n)
+{
+    if (!n) return;
+    if (n->tree && n->nodeNum >= 0 && n->nodeNum < n->tree->nNodes && n->tree->nodes[n->nodeNum] == n)
+        n->tree->nodes[n->nodeNum] = nullptr;
+    delete n;
+}
+
+int p4g_setNodeRelation(p4g_node *n, int relation, int relNum)
+{
+    clearError();
+    if (!n || relNum >= n->tree->nNodes || relNum < -1) {
+        setError("setNodeRelation: bad relative");
+        return 1;
+    }
+    if (relation == 0)
+        n->parent = relNum;
+    else if (relation == 1)
+        n->leftChild = relNum;
+    else if (relation == 2)
+        n->sibling = relNum;
+    else {
+        setError("Error in p4_setNodeRelation: \"relation\" is out of range");
+        return 1;
+    }
+    return 0;
+}
+
+int p4g_setTreeRoot(p4g_tree *t, p4g_node *n)
+{
+    clearError();
+    if (!t || !n) return 1;
+    t->root = n->nodeNum;
+    return 0;
+}
+
+int p4g_setBrLen(p4g_node *n, double len)
+{
+    if (!n) return 1;
+    n->brLen = len;
+    return 0;
+}
+
+double p4g_getBrLen(const p4g_node *n) { return n ? n->brLen : NAN; }
+
+double p4g_getTreeLen(const p4g_tree *t)
+{
+    if (!t) return NAN;
+    double s = 0.0;
+    for (int j = 0; j < t->nNodes; j++) {
+        const int i = t->preOrder[j];
+        if (i == NO_ORDER || i == t->root) continue;
+        s += t->nodes[i]->brLen;
+    }
+    return s;
+}
+
+static int setNodeNum(p4g_node *n, int pNum, int val, int which)
+{
+    clearError();
+    if (!n || pNum < 0 || pNum >= n->tree->nParts) {
+        setError("set model number: bad part");
+        return 1;
+    }
+    (which == 0 ? n->compNums : which == 1 ? n->rMatrixNums : n->gdasrvNums)[pNum] = val;
+    return 0;
+}
+
+int p4g_setCompNum(p4g_node *n, int pNum, int val) { return setNodeNum(n, pNum, val, 0); }
+int p4g_setRMatrixNum(p4g_node *n, int pNum, int val) { return setNodeNum(n, pNum, val, 1); }
+int p4g_setGdasrvNum(p4g_node *n, int pNum, int val) { return setNodeNum(n, pNum, val, 2); }
+
+int p4g_setPrecision(p4g_tree *t, int fp32)
+{
+    clearError();
+    if (!t) return 1;
+    t->precision = fp32 ? FP32 : FP64;
+    return 0;
+}
+
+int p4g_getPrecision(const p4g_tree *t) { return t ? t->precision : -1; }
+
+// ----------------------------------------------------------- evaluation
+int p4g_setPrams(p4g_tree *t, int pNum)
+{
+    clearError();
+    if (!t) return 1;
+    if (!treeReady(t)) return 1;
+    if (pNum >= t->nParts) {
+        setError("setPrams: part out of range");
+        return 1;
+    }
+    for (int p = 0; p < t->nParts; p++) {
+        if (pNum >= 0 && p != pNum) continue;
+        if (!setPramsPart(t, p)) return 1;
+    }
+    return 0;
+}
+
+int p4g_treeLogLike(p4g_tree *t, int getSiteLikes, double *lnL)
+{
+    clearError();
+    if (!t || !lnL) return 1;
+    if (!treeReady(t)) return 1;
+    double sum = 0.0;
+    for (int p = 0; p < t->nParts; p++) {
+        double v = 0.0;
+        if (!evalPart(t, p, true, getSiteLikes != 0, &v)) return 1;
+        sum += v;
+    }
+    t->logLike = sum;
+    *lnL = sum;
+    return 0;
+}
+
+int p4g_partLogLike(p4g_tree *t, p4g_part *p, int pNum, int getSiteLikes, double *lnL)
+{
+    clearError();
+    if (!t || !lnL || pNum < 0 || pNum >= t->nParts) {
+        setError("partLogLike: bad arguments");
+        return 1;
+    }
+    if (p && t->data->parts[pNum] != p) {
+        setError("partLogLike: part is not part pNum of the tree's data");
+        return 1;
+    }
+    if (!treeReady(t)) return 1;
+    return evalPart(t, pNum, false, getSiteLikes != 0, lnL) ? 0 : 1;
+}
+
+int p4g_calculateBigPDecksPart(p4g_node *n, int pNum)
+{
+    clearError();
+    if (!n) return 1;
+    Tree *t = n->tree;
+    if (!treeReady(t)) return 1;
+    return bigPDecksPart(t, n, pNum) ? 0 : 1;
+}
+
+int p4g_calculateBigPDecks(p4g_node *n)
+{
+    clearError();
+    if (!n) return 1;
+    for (int p = 0; p < n->tree->nParts; p++)
+        if (p4g_calculateBigPDecksPart(n, p)) return 1;
+    return 0;
+}
+
+int p4g_calculateAllBigPDecksAllParts(p4g_tree *t)
+{
+    clearError();
+    if (!t || !treeReady(t)) return 1;
+    for (int p = 0; p < t->nParts; p++) {
+        std::vector<int> nodes;
+        ModelPart *mp = t->model->parts[p];
+        for (int n = 0; n < t->nNodes; n++) {
+            if (n == t->root) continue;
+            Node *nd = t->nodes[n];
+            if (!checkNodeModel(t, mp, nd, p)) return 1;
+            const int slot = nd->compNums[p] * mp->nRMatrices + nd->rMatrixNums[p];
+            if (mp->bQETneedsReset && mp->bQETneedsReset[slot]) {
+                if (!runEig(t, p, {slot})) return 1;
+                mp->bQETneedsReset[slot] = 0;
+            }
+            nodes.push_back(n);
+        }
+        if (!runPmat(t, p, nodes)) return 1;
+    }
+    return 0;
+}
+
+int p4g_setConditionalLikelihoodsOfInternalNodePart(p4g_node *n, int pNum)
+{
+    clearError();
+    if (!n || pNum < 0 || pNum >= n->tree->nParts) return 1;
+    Tree *t = n->tree;
+    TreePart &tp = t->tp[pNum];
+    if (tp.dirty.size() != (size_t)t->nNodes) tp.dirty.assign(t->nNodes, 1);
+    tp.dirty[n->nodeNum] = 1;
+    return 0;
+}
+
+int p4g_setConditionalLikelihoodsOfInternalNode(p4g_node *n)
+{
+    if (!n) return 1;
+    for (int p = 0; p < n->tree->nParts; p++)
+        if (p4g_setConditionalLikelihoodsOfInternalNodePart(n, p)) return 1;
+    return 0;
+}
+
+// ---------------------------------------------------------- state sync
+static bool sameShape(Tree *a, Tree *b)
+{
+    if (a->nNodes != b->nNodes || a->nParts != b->nParts) {
+        setError("trees differ in shape");
+        return false;
+    }
+    return true;
+}
+
+int p4g_copyCondLikes(p4g_tree *a, p4g_tree *b, int doAll)
+{
+    clearError();
+    if (!a || !b || !sameShape(a, b) || !treeReady(a) || !treeReady(b)) return 1;
+    if (a->precision != b->precision) {
+        setError("copyCondLikes: trees have different precision");
+        return 1;
+    }
+    hipStream_t s = stream();
+    for (int p = 0; p < a->nParts; p++) {
+        TreePart &A = a->tp[p], &B = b->tp[p];
+        if (A.nSlots != B.nSlots || A.clBytes != B.clBytes) {
+            setError("copyCondLikes: partial buffers differ");
+            return 1;
+        }
+        if (doAll) {
+            if (!hipOk(hipMemcpyAsync(B.cl, A.cl, A.clBytes * A.nSlots, hipMemcpyDeviceToDevice, s), "copy partials"))
+                return 1;
+            if (A.scl && !hipOk(hipMemcpyAsync(B.scl, A.scl, sizeof(int) * (size_t)A.nSlots * A.nPatPad,
+                                               hipMemcpyDeviceToDevice, s),
+                                "copy scale exponents"))
+                return 1;
+            B.dirty = A.dirty;
+        } else {
+            for (int j = 0; j < a->nNodes; j++) {
+                const int i = a->preOrder[j];
+                if (i == NO_ORDER || a->nodes[i]->isLeaf) continue;
+                if (!(A.dirty[i] || B.dirty[i])) continue;
+                const int sl = a->slotOf[i];
+                if (!hipOk(hipMemcpyAsync((char *)B.cl + A.clBytes * sl, (char *)A.cl + A.clBytes * sl, A.clBytes,
+                                          hipMemcpyDeviceToDevice, s),
+                           "copy partials"))
+                    return 1;
+                if (A.scl && !hipOk(hipMemcpyAsync(B.scl + (size_t)sl * A.nPatPad, A.scl + (size_t)sl * A.nPatPad,
+                                                   sizeof(int) * A.nPatPad, hipMemcpyDeviceToDevice, s),
+                                    "copy scale exponents"))
+                    return 1;
+                B.dirty[i] = A.dirty[i];
+            }
+        }
+        B.everComputed = A.everComputed;
+    }
+    return 0;
+}
+
+int p4g_copyBigPDecks(p4g_tree *a, p4g_tree *b, int doAll)
+{
+    clearError();
+    if (!doAll) {
+        setError("p4_copyBigPDecks() doAll is not set. Programming error?");
+        return 1;
+    }
+    if (!a || !b || !sameShape(a, b) || !treeReady(a) || !treeReady(b)) return 1;
+    hipStream_t s = stream();
+    for (int p = 0; p < a->nParts; p++) {
+        TreePart &A = a->tp[p], &B = b->tp[p];
+        const size_t n = (size_t)a->nNodes * A.K * A.dim * A.dim;
+        if (!hipOk(hipMemcpyAsync(B.PT64, A.PT64, n * 8, hipMemcpyDeviceToDevice, s), "copy P") ||
+            !hipOk(hipMemcpyAsync(B.PT32, A.PT32, n * 4, hipMemcpyDeviceToDevice, s), "copy P"))
+            return 1;
+        B.pValid = A.pValid;
+    }
+    return 0;
+}
+
+// Pf/p4_treeCopyVerify.c:63-180
+int p4g_copyModelPrams(p4g_tree *a, p4g_tree *b)
+{
+    clearError();
+    if (!a || !b || a->model->nParts != b->model->nParts) return 1;
+    hipStream_t s = stream();
+    for (int p = 0; p < a->model->nParts; p++) {
+        ModelPart *A = a->model->parts[p], *B = b->model->parts[p];
+        const int dim = A->dim;
+        for (int c = 0; c < A->nComps; c++)
+            if (A->comps[c].free && A->comps[c].val != B->comps[c].val)
+                std::memcpy(B->comps[c].val, A->comps[c].val, sizeof(double) * dim);
+        for (int r = 0; r < A->nRMatrices; r++)
+            if (A->rMatrices[r].free) {
+                B->rMatrices[r].bigR = A->rMatrices[r].bigR;
+                B->rMatrices[r].kappa = A->rMatrices[r].kappa;
+            }
+        TreePart &TA = a->tp[p], &TB = b->tp[p];
+        if (TA.alloc && TB.alloc && TA.nEig == TB.nEig) {
+            const size_t ne = (size_t)TA.nEig;
+            if (!hipOk(hipMemcpyAsync(TB.eigV, TA.eigV, ne * dim * dim * 8, hipMemcpyDeviceToDevice, s), "copy eig") ||
+                !hipOk(hipMemcpyAsync(TB.eigVi, TA.eigVi, ne * dim * dim * 8, hipMemcpyDeviceToDevice, s), "copy eig") ||
+                !hipOk(hipMemcpyAsync(TB.eigLam, TA.eigLam, ne * dim * 8, hipMemcpyDeviceToDevice, s), "copy eig"))
+                return 1;
+        }
+        for (int g = 0; g < A->nGdasrvs; g++)
+            if (A->gdasrvs[g]->free && A->gdasrvs[g] != B->gdasrvs[g]) {
+                B->gdasrvs[g]->val[0] = A->gdasrvs[g]->val[0];
+                for (int k = 0; k < A->nCat; k++) B->gdasrvs[g]->rates[k] = A->gdasrvs[g]->rates[k];
+            }
+        if (A->pInvarFree) B->pInvar = A->pInvar;
+        if (a->model->doRelRates && a->model->relRatesAreFree) B->relRate = A->relRate;
+    }
+    return 0;
+}
+
+int p4g_verifyIdentityOfTwoTrees(p4g_tree *a, p4g_tree *b)
+{
+    clearError();
+    if (!a || !b || !sameShape(a, b) || !treeReady(a) || !treeReady(b)) return 1;
+    if (!hipOk(hipStreamSynchronize(stream()), "sync")) return 1;
+    for (int p = 0; p < a->nParts; p++) {
+        TreePart &A = a->tp[p], &B = b->tp[p];
+        const size_t n = (size_t)a->nNodes * A.K * A.dim * A.dim;
+        std::vector<double> pa(n), pb(n);
+        if (!hipOk(hipMemcpy(pa.data(), A.PT64, n * 8, hipMemcpyDeviceToHost), "copy P") ||
+            !hipOk(hipMemcpy(pb.data(), B.PT64, n * 8, hipMemcpyDeviceToHost), "copy P"))
+            return 1;
+        for (int node = 0; node < a->nNodes; node++) {
+            if (node == a->root || !A.pValid[node] || !B.pValid[node]) continue;
+            for (size_t i = (size_t)node * A.K * A.dim * A.dim; i < (size_t)(node + 1) * A.K * A.dim * A.dim; i++)
+                if (std::fabs(pa[i] - pb[i]) > 1e-15) return 1;
+        }
+        if (A.clBytes != B.clBytes) return 1;
+        const size_t nb = A.clBytes * A.nSlots;
+        std::vector<char> ca(nb), cb(nb);
+        if (!hipOk(hipMemcpy(ca.data(), A.cl, nb, hipMemcpyDeviceToHost), "copy partials") ||
+            !hipOk(hipMemcpy(cb.data(), B.cl, nb, hipMemcpyDeviceToHost), "copy partials"))
+            return 1;
+        const size_t per = (size_t)A.nPatPad * A.K * A.dim;
+        for (int node = 0; node < a->nNodes; node++) {
+            const int sl = a->slotOf[node];
+            if (sl < 0 || node == a->root) continue;
+            if (a->precision == FP32) {
+                const float *fa = (const float *)ca.data() + per * sl, *fb = (const float *)cb.data() + per * sl;
+                for (size_t i = 0; i < per; i++)
+                    if (std::fabs((double)fa[i] - (double)fb[i]) > 1e-15) return 1;
+            } else {
+                const double *da = (const double *)ca.data() + per * sl, *db = (const double *)cb.data() + per * sl;
+                for (size_t i = 0; i < per; i++)
+                    if (std::fabs(da[i] - db[i]) > 1e-15) return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------ inspection
+int p4g_getBigPDeck(const p4g_node *n, int pNum, double *out)
+{
+    clearError();
+    if (!n || !out || pNum < 0 || pNum >= n->tree->nParts) return 1;
+    const TreePart &tp = n->tree->tp[pNum];
+    if (!tp.alloc || !tp.pValid[n->nodeNum]) {
+        setError("getBigPDeck: P not computed");
+        return 1;
+    }
+    const int K = tp.K, S = tp.dim;
+    std::vector<double> h((size_t)K * S * S);
+    if (!hipOk(hipStreamSynchronize(stream()), "sync") ||
+        !hipOk(hipMemcpy(h.data(), tp.PT64 + (size_t)n->nodeNum * K * S * S, h.size() * 8, hipMemcpyDeviceToHost),
+               "copy P"))
+        return 1;
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < S; i++)
+            for (int j = 0; j < S; j++) out[((size_t)k * S + i) * S + j] = h[((size_t)k * S + j) * S + i];
+    return 0;
+}
+
+int p4g_getCondLikes(const p4g_node *n, int pNum, double *out)
+{
+    clearError();
+    if (!n || !out || pNum < 0 || pNum >= n->tree->nParts) return 1;
+    const Tree *t = n->tree;
+    const TreePart &tp = t->tp[pNum];
+    const int sl = t->slotOf.empty() ? -1 : t->slotOf[n->nodeNum];
+    if (!tp.alloc || sl < 0 || n->nodeNum == t->root) {
+        setError("getCondLikes: no stored partials for this node (leaf or root)");
+        return 1;
+    }
+    const int K = tp.K, S = tp.dim, nP = tp.nPat;
+    const size_t per = (size_t)tp.nPatPad * K * S;
+    std::vector<int> e;
+    if (!hipOk(hipStreamSynchronize(stream()), "sync")) return 1;
+    std::vector<double> v(per);
+    if (t->precision == FP32) {
+        std::vector<float> f(per);
+        e.resize(tp.nPatPad);
+        if (!hipOk(hipMemcpy(f.data(), (const char *)tp.cl + tp.clBytes * sl, per * 4, hipMemcpyDeviceToHost), "copy") ||
+            !hipOk(hipMemcpy(e.data(), tp.scl + (size_t)sl * tp.nPatPad, e.size() * 4, hipMemcpyDeviceToHost), "copy"))
+            return 1;
+        for (size_t i = 0; i < per; i++) v[i] = f[i];
+    } else if (!hipOk(hipMemcpy(v.data(), (const char *)tp.cl + tp.clBytes * sl, per * 8, hipMemcpyDeviceToHost), "copy"))
+        return 1;
+    for (int k = 0; k < K; k++)
+        for (int s = 0; s < S; s++)
+            for (int p = 0; p < nP; p++) {
+                double x = v[((size_t)p * K + k) * S + s];
+                if (!e.empty()) x = std::ldexp(x, -e[p]);
+                out[((size_t)k * S + s) * nP + p] = x;
+            }
+    return 0;
+}
+
+// -------------------------------------------------------------- batches
+}  // extern "C"
+
+struct p4g_batch {
+    std::vector<Tree *> trees;
+    PruneArgs *dArgs = nullptr;
+    int *dBlockItem = nullptr;
+    PruneArgs *hArgs = nullptr;
+    int *hBlockItem = nullptr;
+    int capItems = 0, capBlocks = 0;
+    double *hRes = nullptr;
+    ~p4g_batch()
+    {
+        devFree(dArgs);
+        devFree(dBlockItem);
+        hostFree(hArgs);
+        hostFree(hBlockItem);
+        hostFree(hRes);
+    }
+};
+
+extern "C" {
+
+p4g_batch *p4g_newBatch(int nTrees, p4g_tree **trees)
+{
+    clearError();
+    if (nTrees < 1 || !trees) {
+        setError("newBatch: bad arguments");
+        return nullptr;
+    }
+    auto *b = new p4g_batch();
+    for (int i = 0; i < nTrees; i++) b->trees.push_back(trees[i]);
+    return b;
+}
+
+void p4g_freeBatch(p4g_batch *b) { delete b; }
+
+// All parts of all trees: pending partials + site sums, one launch per
+// (state count, precision) group.
+int p4g_batchLogLike(p4g_batch *b, double *lnL)
+{
+    clearError();
+    if (!b || !lnL) return 1;
+    struct Item {
+        Tree *t;
+        int p, nOps, tree;
+    };
+    std::vector<Item> items;
+    for (size_t i = 0; i < b->trees.size(); i++) {
+        Tree *t = b->trees[i];
+        if (!treeReady(t)) return 1;
+        for (int p = 0; p < t->nParts; p++) {
+            int nOps = 0;
+            if (!stagePart(t, p, false, &nOps)) return 1;
+            items.push_back({t, p, nOps, (int)i});
+        }
+    }
+    const int nItems = (int)items.size();
+    int nBlocksTot = 0;
+    for (auto &it : items) nBlocksTot += it.t->tp[it.p].nPatPad / pruneBlockPatterns(it.t->model->parts[it.p]->nCat);
+    if (nItems > b->capItems) {
+        devFree(b->dArgs);
+        hostFree(b->hArgs);
+        hostFree(b->hRes);
+        b->capItems = nItems * 2;
+        if (!devAlloc(&b->dArgs, b->capItems, "hipMalloc batch args") ||
+            !hostAlloc(&b->hArgs, b->capItems, "hipHostMalloc batch args") ||
+            !hostAlloc(&b->hRes, (size_t)b->capItems * 2, "hipHostMalloc batch results"))
+            return 1;
+    }
+    if (nBlocksTot > b->capBlocks) {
+        devFree(b->dBlockItem);
+        hostFree(b->hBlockItem);
+        b->capBlocks = nBlocksTot * 2;
+        if (!devAlloc(&b->dBlockItem, (size_t)b->capBlocks * 2, "hipMalloc batch map") ||
+            !hostAlloc(&b->hBlockItem, (size_t)b->capBlocks * 2, "hipHostMalloc batch map"))
+            return 1;
+    }
+    // group by (dim, precision): items of a group are contiguous in hArgs
+    std::vector<int> order(nItems);
+    for (int i = 0; i < nItems; i++) order[i] = i;
+    auto key = [&](int i) { return items[i].t->model->parts[items[i].p]->dim * 2 + items[i].t->precision; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
+    for (int j = 0; j < nItems; j++) {
+        const Item &it = items[order[j]];
+        fillPruneArgs(it.t, it.p, it.nOps, b->hArgs[j], false);
+    }
+    hipStream_t s = stream();
+    if (!hipOk(hipMemcpyAsync(b->dArgs, b->hArgs, sizeof(PruneArgs) * nItems, hipMemcpyHostToDevice, s),
+               "stage batch args"))
+        return 1;
+    int j = 0, blk = 0;
+    std::vector<std::pair<int, int>> launches;  // (first block, count) per group
+    std::vector<int> groupDim, groupPrec;
+    while (j < nItems) {
+        const int k0 = key(order[j]);
+        const int first = blk;
+        int jj = j;
+        while (jj < nItems && key(order[jj]) == k0) {
+            const Item &it = items[order[jj]];
+            const int nb = it.t->tp[it.p].nPatPad / pruneBlockPatterns(it.t->model->parts[it.p]->nCat);
+            for (int q = 0; q < nb; q++) {
+                b->hBlockItem[2 * blk] = jj;
+                b->hBlockItem[2 * blk + 1] = q;
+                blk++;
+            }
+            jj++;
+        }
+        launches.push_back({first, blk - first});
+        groupDim.push_back(items[order[j]].t->model->parts[items[order[j]].p]->dim);
+        groupPrec.push_back(items[order[j]].t->precision);
+        j = jj;
+    }
+    if (!hipOk(hipMemcpyAsync(b->dBlockItem, b->hBlockItem, sizeof(int) * 2 * blk, hipMemcpyHostToDevice, s),
+               "stage batch map"))
+        return 1;
+    for (size_t g = 0; g < launches.size(); g++)
+        if (!hipOk(launchPruneBatch(b->dArgs, b->dBlockItem + 2 * launches[g].first, launches[g].second,
+                                    groupDim[g], groupPrec[g], s),
+                   "launch batched pruning kernel"))
+            return 1;
+    for (int q = 0; q < nItems; q++) {
+        const Item &it = items[order[q]];
+        if (!hipOk(hipMemcpyAsync(b->hRes + 2 * q, it.t->tp[it.p].result, 2 * sizeof(double),
+                                  hipMemcpyDeviceToHost, s),
+                   "copy batch results"))
+            return 1;
+    }
+    if (!hipOk(hipStreamSynchronize(s), "batched pruning kernel")) return 1;
+    for (size_t i = 0; i < b->trees.size(); i++) lnL[i] = 0.0;
+    for (int q = 0; q < nItems; q++) {
+        const Item &it = items[order[q]];
+        const double v = b->hRes[2 * q];
+        lnL[it.tree] += v;
+        finishPart(it.t, it.p, false, v, b->hRes[2 * q + 1] != 0.0);
+    }
+    for (size_t i = 0; i < b->trees.size(); i++) b->trees[i]->logLike = lnL[i];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// p4g_internal.h -- host-side state of libp4gpu and the argument blocks of
+// its HIP kernels.  Not part of the public C-ABI (include/p4gpu.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace p4g {
+
+// Character codes of coded sequences (Pf/defines.h:33-36).
+constexpr int GAP_CODE = -1;
+constexpr int QMARK_CODE = -2;
+constexpr int N_LIKE = -3;
+constexpr int EQUATES_BASE = -64;
+constexpr int NO_ORDER = -10000;
+
+// rMatrix spec ids (Pf/defines.h:1-30).
+constexpr int RMATRIX_2P = 5;
+constexpr int RMATRIX_SPECIFIED = 20;
+constexpr int RMATRIX_POISSON = 100;
+
+// Device tip-code alphabet: 0..dim-1 a state, TIP_ALL every state (gap, '?',
+// or an equate allowing every state), dim..TIP_ALL-1 a partial ambiguity.
+constexpr unsigned TIP_ALL = 255;
+constexpr int MAX_DIM = 64;
+constexpr int MAX_CHILDREN = 64;
+
+// Tree precision modes.
+enum Precision { FP64 = 0, FP32 = 1 };
+
+void setError(const std::string &msg);
+void clearError();
+bool hipOk(hipError_t e, const char *what);
+hipStream_t stream();
+bool deviceReady();
+
+// ------------------------------------------------------------------ data
+struct Part;
+struct Data {
+    int nTax = 0;
+    int nParts = 0;
+    std::vector<Part *> parts;
+};
+
+// Device copy of one data part's patterns, laid out for the pruning kernel.
+struct DevPart {
+    bool valid = false;
+    int nPatPad = 0;       // padded to a multiple of 64 patterns
+    uint8_t *tips = nullptr;      // [nTax][nPatPad] device tip codes
+    double *counts = nullptr;     // [nPatPad] pattern weights (0 for padding)
+    uint32_t *invMask = nullptr;  // [nWords][nPatPad] states compatible with every taxon
+    uint64_t *ambMask = nullptr;  // [nAmb] state masks of partial ambiguities
+    int nAmb = 0;
+    int nWords = 0;
+    std::vector<uint64_t> hostAmb;
+};
+
+struct Part {
+    Data *data = nullptr;
+    int dim = 0, nTax = 0, nChar = 0, nEquates = 0;
+    std::string symbols, equateSymbols;
+    std::vector<int> equates;       // [nEquates][dim]
+    std::vector<int> sequences;     // [nTax][nChar]
+    int nPatterns = 0;
+    std::vector<int> patterns;      // [nTax][nPatterns]
+    std::vector<int> patternCounts; // [nPatterns]
+    std::vector<int> seqPosPatternIndex; // [nChar]
+    std::vector<int> invarVec;      // [nPatterns]
+    std::vector<int> invarArray;    // [dim][nPatterns]
+    bool invarSet = false;
+    std::vector<double> siteLikes;  // [nChar]
+    DevPart dev;
+    ~Part();
+};
+
+// ----------------------------------------------------------------- model
+struct Comp {
+    int free = 0;
+    double *val = nullptr;  // aliased host array [dim]
+};
+struct RMatrix {
+    int free = 0, spec = RMATRIX_POISSON;
+    std::vector<double> bigR;  // [dim][dim]
+    double kappa = 2.0;
+};
+struct Gdasrv {
+    int free = 0, nCat = 1;
+    double *val = nullptr, *freqs = nullptr, *rates = nullptr;  // aliased
+};
+struct ModelPart {
+    int dim = 0, nComps = 0, nRMatrices = 0, nGdasrvs = 0, nCat = 1, pInvarFree = 0;
+    std::vector<Comp> comps;
+    std::vector<RMatrix> rMatrices;
+    std::vector<Gdasrv *> gdasrvs;
+    double pInvar = 0.0, relRate = 1.0;
+    int *bQETneedsReset = nullptr;  // aliased [nComps][nRMatrices]
+    ~ModelPart();
+};
+struct Model {
+    int nParts = 0, doRelRates = 0, relRatesAreFree = 0, nFreePrams = 0, isHet = 0;
+    int *rMatrixNormalizeTo1 = nullptr;
+    double *PINVAR_MIN = nullptr, *PINVAR_MAX = nullptr, *KAPPA_MIN = nullptr,
+           *KAPPA_MAX = nullptr, *GAMMA_SHAPE_MIN = nullptr, *GAMMA_SHAPE_MAX = nullptr,
+           *PIVEC_MIN = nullptr, *PIVEC_MAX = nullptr, *RATE_MIN = nullptr, *RATE_MAX = nullptr,
+           *RELRATE_MIN = nullptr, *RELRATE_MAX = nullptr, *BRLEN_MIN = nullptr,
+           *BRLEN_MAX = nullptr;
+    std::vector<ModelPart *> parts;
+    ~Model();
+};
+
+// ------------------------------------------------------------------ tree
+struct Tree;
+struct Node {
+    int nodeNum = 0;
+    Tree *tree = nullptr;
+    int parent = -1, leftChild = -1, sibling = -1;
+    int seqNum = -1, isLeaf = 0, inTree = 1;
+    double brLen = -1.0;
+    std::vector<int> compNums, rMatrixNums, gdasrvNums;
+};
+
+// One 4-int program record: an internal node to combine, and its children.
+struct OpRec {
+    int dst;     // cl slot written (root: unused)
+    int flags;   // bit0: root (fused site sum)
+    int nKids;
+    int kidStart;
+};
+struct KidRec {
+    int kind;    // 0 leaf, 1 internal
+    int idx;     // leaf: tip row (seqNum); internal: cl slot
+    int pnode;   // node whose P matrices (branch above it) are used
+    int pad;
+};
+constexpr int OP_ROOT = 1;
+
+// Per-(tree, part) device state.
+struct TreePart {
+    bool alloc = false;
+    int dim = 0, K = 0, nPat = 0, nPatPad = 0, nSlots = 0;
+    int precision = FP64;
+    // Eigen slots: [nComps*nRMat]; input pi[dim] + R[dim*dim], output
+    // V[dim*dim], Vinv[dim*dim], lam[dim].
+    int nEig = 0;
+    double *eigIn = nullptr, *eigV = nullptr, *eigVi = nullptr, *eigLam = nullptr;
+    // P^T per node and category: [nNodes][K][dim(j)][dim(s)] = P[s][j]
+    double *PT64 = nullptr;
+    float *PT32 = nullptr;
+    // Partials: [nSlots][nPatPad][K][dim] of float or double, scale
+    // exponents [nSlots][nPatPad] (FP32 mode).
+    void *cl = nullptr;
+    int *scl = nullptr;
+    size_t clBytes = 0;
+    // Reduction state.
+    double *blockSums = nullptr;
+    int *blockBad = nullptr;
+    unsigned *counter = nullptr;
+    double *result = nullptr;      // [2] lnL, bad flag
+    double *patLikes = nullptr;    // [nPatPad]
+    int maxBlocks = 0;
+    // Programs.
+    int *prog = nullptr;           // device ops+kids
+    int progCap = 0;
+    double *rootPi = nullptr;      // [dim]
+    // Host-side flags.
+    std::vector<char> dirty;       // per node
+    std::vector<char> pValid;      // per node
+    bool everComputed = false;
+    // Staging: pinned host and a device mirror of the same size.
+    double *hStage = nullptr;
+    double *dStage = nullptr;
+    size_t hStageCap = 0;
+    int *hProg = nullptr;
+    int hProgCap = 0;
+    double *hResult = nullptr;     // pinned [2]
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double lastKernelUs = -1.0;
+};
+
+struct Tree {
+    int nNodes = 0, nLeaves = 0;
+    std::vector<Node *> nodes;
+    int root = -1;
+    int *preOrder = nullptr, *postOrder = nullptr;  // aliased
+    double *partLikes = nullptr;                    // aliased
+    Data *data = nullptr;
+    Model *model = nullptr;
+    int nParts = 0;
+    double logLike = 0.0;
+    int precision = FP64;
+    std::vector<TreePart> tp;
+    std::vector<int> slotOf;  // node -> cl slot (-1 none)
+    ~Tree();
+};
+
+// ------------------------------------------------------- kernel arguments
+struct EigArgs {
+    int dim;
+    int nSlots;
+    const int *slotList;  // which eigen slots to compute
+    const double *in;     // per slot: pi[dim], R[dim*dim]
+    double *V, *Vi, *lam;
+};
+
+struct PmatArgs {
+    int dim, K, nItems;     // items = listed nodes
+    const int *nodeList;    // [nItems] node numbers
+    const int *eigSlot;     // [nItems]
+    const double *t;        // [nItems][K] effective branch lengths
+    const double *V, *Vi, *lam;
+    double *PT64;
+    float *PT32;
+};
+
+struct PruneArgs {
+    const OpRec *ops;
+    const KidRec *kids;
+    int nOps;
+    int dim, K, log2Kp;
+    int nPat, nPatPad;
+    const uint8_t *tips;
+    const void *PT;          // float or double by precision
+    void *cl;
+    int *scl;
+    const double *counts;
+    const uint32_t *invMask;
+    int nWords;
+    const uint64_t *ambMask;
+    const double *rootPi;
+    double pInvar;
+    int rootLeafRow;         // tip row of a leaf root, else -1
+    double *blockSums;
+    int *blockBad;
+    unsigned *counter;
+    double *result;
+    double *patLikes;        // may be null
+};
+
+// Kernel launchers (p4g_kernels.hip).
+hipError_t launchEig(const EigArgs &a, hipStream_t s);
+hipError_t launchPmat(const PmatArgs &a, hipStream_t s);
+hipError_t launchPrune(const PruneArgs &a, int precision, int nBlocks, hipStream_t s);
+hipError_t launchPruneBatch(const PruneArgs *dArgs, const int *dBlockItem, int nBlocks,
+                            int dim, int precision, hipStream_t s);
+int pruneBlockPatterns(int K);  // patterns per 256-thread block
+
+// Host numerics (p4g_host.hip).
+int discreteGamma(double *freqK, double *rK, double alfa, double beta, int K, int median);
+
+}  // namespace p4g

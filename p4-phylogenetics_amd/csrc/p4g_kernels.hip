@@ -1,0 +1,544 @@
+// p4g_kernels.hip -- CDNA4 (gfx950) kernels of the likelihood engine.
+//
+//  k_eig    one 64-lane workgroup per (comp, rMatrix) eigensystem: builds Q
+//           (Pf/util.c:44-115), symmetrises it with pi^(1/2) and runs a
+//           parallel cyclic Jacobi sweep (round-robin pairing, all disjoint
+//           rotations of a round applied at once) in FP64.  Replaces the
+//           EISPACK route of Pf/eig.c:63-161.
+//  k_pmat   one thread per (branch, category, i, j) entry:
+//           P = I + V diag(expm1(lam t)) Vinv in FP64 (Pf/eig.c:163-223
+//           computes V diag(exp(lam t)) Vinv; the expm1 form keeps short
+//           branches accurate), stored transposed in FP64 and FP32.
+//  k_prune  the whole postorder in ONE launch: every workgroup owns a tile
+//           of site patterns and walks the node program for that tile, so
+//           no inter-workgroup dependency (patterns are independent) and no
+//           per-level launches.  Lane = (pattern, gamma category); partials
+//           are [slot][pattern][cat][state], so a wavefront reads and writes
+//           contiguous runs (1 KiB per instruction for FP32 DNA).  The root
+//           node is fused with the weighted site-log-likelihood sum
+//           (Pf/p4_tree.c:1029-1378) and a last-workgroup compensated FP64
+//           reduction.  FP32 mode rescales a (node, pattern) vector by an
+//           exact power of two when its max over categories and states
+//           drops below 2^-24, accumulating integer exponents per node.
+#include "p4g_internal.h"
+
+#include <cmath>
+
+namespace p4g {
+
+// ------------------------------------------------------------------- eigen
+__device__ __forceinline__ int rrPlayer(int pos, int r, int n)
+{
+    return pos == 0 ? 0 : ((pos - 1 + r) % (n - 1)) + 1;
+}
+
+__device__ __forceinline__ double waveSum(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__global__ __launch_bounds__(64) void k_eig(EigArgs a)
+{
+    extern __shared__ double sm[];
+    const int S = a.dim;
+    const int n = (S + 1) & ~1;
+    const int h = n / 2;
+    double *A = sm;             // [n][n]
+    double *U = A + n * n;      // [n][n]
+    double *sq = U + n * n;     // [n]
+    double *cs = sq + n;        // [h][2]
+    int *pq = (int *)(cs + n);  // [h][2]
+    const int slot = a.slotList[blockIdx.x];
+    const double *pi = a.in + (size_t)slot * (S + S * S);
+    const double *R = pi + S;
+    const int lane = threadIdx.x;
+
+    for (int i = lane; i < n * n; i += 64) {
+        A[i] = 0.0;
+        U[i] = ((i / n) == (i % n)) ? 1.0 : 0.0;
+    }
+    for (int i = lane; i < n; i += 64) sq[i] = (i < S) ? sqrt(pi[i]) : 1.0;
+    __syncthreads();
+    if (lane == 0) {
+        // Q = R . diag(pi), diagonal = -row sum, scaled to mean rate 1,
+        // in the operation order of Pf/util.c:44-115.  Q is built in U's
+        // storage and then symmetrised into A.
+        double *Q = U;
+        for (int c = 0; c < S; c++)
+            for (int r = 0; r < S; r++) Q[r * n + c] = R[r * S + c] * pi[c];
+        for (int r = 0; r < S; r++) {
+            double sum = 0.0;
+            for (int c = 0; c < S; c++)
+                if (r != c) sum = sum + Q[r * n + c];
+            Q[r * n + r] = -sum;
+        }
+        double sumODE = 0.0;
+        for (int r = 0; r < S; r++)
+            for (int c = 0; c < S; c++)
+                if (r != c) sumODE = sumODE + pi[r] * Q[r * n + c];
+        sumODE = 1.0 / sumODE;
+        for (int r = 0; r < S; r++)
+            for (int c = 0; c < S; c++) Q[r * n + c] = Q[r * n + c] * sumODE;
+        for (int i = 0; i < S; i++)
+            for (int j = 0; j < S; j++) {
+                if (i == j) {
+                    A[i * n + i] = Q[i * n + i];
+                } else {
+                    double aij = sq[i] * Q[i * n + j] / sq[j];
+                    double aji = sq[j] * Q[j * n + i] / sq[i];
+                    A[i * n + j] = 0.5 * (aij + aji);
+                }
+            }
+        for (int i = 0; i < n * n; i++) U[i] = ((i / n) == (i % n)) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, dg = 0.0;
+        for (int idx = lane; idx < S * S; idx += 64) {
+            int i = idx / S, j = idx % S;
+            double v = A[i * n + j];
+            if (i < j) off += v * v;
+            if (i == j) dg += v * v;
+        }
+        off = waveSum(off);
+        dg = waveSum(dg);
+        if (off == 0.0 || off < 1e-34 * dg) break;
+        for (int r = 0; r < n - 1; r++) {
+            if (lane < h) {
+                int p = rrPlayer(lane, r, n), q = rrPlayer(n - 1 - lane, r, n);
+                if (p > q) {
+                    int t = p;
+                    p = q;
+                    q = t;
+                }
+                double c = 1.0, s = 0.0;
+                if (q < S) {
+                    double apq = A[p * n + q];
+                    if (apq != 0.0) {
+                        double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                        double t;
+                        if (fabs(theta) > 1e150)
+                            t = 0.5 / theta;
+                        else
+                            t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        s = t * c;
+                    }
+                }
+                pq[2 * lane] = p;
+                pq[2 * lane + 1] = q;
+                cs[2 * lane] = c;
+                cs[2 * lane + 1] = s;
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int it = lane; it < h * S; it += 64) {
+                int pr = it / S, k = it % S;
+                int p = pq[2 * pr], q = pq[2 * pr + 1];
+                double c = cs[2 * pr], s = cs[2 * pr + 1];
+                if (s == 0.0 || q >= S) continue;
+                double apk = A[p * n + k], aqk = A[q * n + k];
+                A[p * n + k] = c * apk - s * aqk;
+                A[q * n + k] = s * apk + c * aqk;
+            }
+            __syncthreads();
+            // columns: A <- A J, U <- U J
+            for (int it = lane; it < h * S; it += 64) {
+                int pr = it / S, k = it % S;
+                int p = pq[2 * pr], q = pq[2 * pr + 1];
+                double c = cs[2 * pr], s = cs[2 * pr + 1];
+                if (s == 0.0 || q >= S) continue;
+                double akp = A[k * n + p], akq = A[k * n + q];
+                A[k * n + p] = c * akp - s * akq;
+                A[k * n + q] = s * akp + c * akq;
+                double ukp = U[k * n + p], ukq = U[k * n + q];
+                U[k * n + p] = c * ukp - s * ukq;
+                U[k * n + q] = s * ukp + c * ukq;
+            }
+            __syncthreads();
+        }
+    }
+    double *V = a.V + (size_t)slot * S * S;
+    double *Vi = a.Vi + (size_t)slot * S * S;
+    double *lam = a.lam + (size_t)slot * S;
+    for (int idx = lane; idx < S * S; idx += 64) {
+        int i = idx / S, k = idx % S;
+        double u = U[i * n + k];
+        V[i * S + k] = u / sq[i];
+        Vi[k * S + i] = u * sq[i];
+    }
+    for (int k = lane; k < S; k += 64) lam[k] = A[k * n + k];
+}
+
+hipError_t launchEig(const EigArgs &a, hipStream_t s)
+{
+    const int n = (a.dim + 1) & ~1;
+    size_t bytes = sizeof(double) * (2 * n * n + 2 * n) + sizeof(int) * n;
+    if (bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_eig,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_eig, dim3(a.nSlots), dim3(64), bytes, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- P matrices
+__global__ __launch_bounds__(256) void k_pmat(PmatArgs a)
+{
+    const int S = a.dim, K = a.K;
+    const long total = (long)a.nItems * K * S * S;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    const int j = gid % S;
+    const int i = (gid / S) % S;
+    const int k = (gid / (S * S)) % K;
+    const int it = gid / ((long)S * S * K);
+    const int node = a.nodeList[it];
+    const int es = a.eigSlot[it];
+    const double t = a.t[(size_t)it * K + k];
+    const double *V = a.V + (size_t)es * S * S;
+    const double *Vi = a.Vi + (size_t)es * S * S;
+    const double *lam = a.lam + (size_t)es * S;
+    double acc = 0.0;
+    for (int m = 0; m < S; m++) acc += V[i * S + m] * Vi[m * S + j] * expm1(lam[m] * t);
+    const double pij = (i == j ? 1.0 : 0.0) + acc;
+    const size_t o = (((size_t)node * K + k) * S + j) * S + i;
+    a.PT64[o] = pij;
+    a.PT32[o] = (float)pij;
+}
+
+hipError_t launchPmat(const PmatArgs &a, hipStream_t s)
+{
+    const long total = (long)a.nItems * a.K * a.dim * a.dim;
+    if (total == 0) return hipSuccess;
+    const int blocks = (int)((total + 255) / 256);
+    hipLaunchKernelGGL(k_pmat, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------------- pruning
+constexpr int PRUNE_THREADS = 256;
+constexpr float SCALE_THRESH = 5.9604645e-08f;  // 2^-24
+
+int pruneBlockPatterns(int K)
+{
+    int Kp = 1;
+    while (Kp < K) Kp <<= 1;
+    return PRUNE_THREADS / Kp;
+}
+
+template <typename T>
+__device__ __forceinline__ T groupMax(T v, int Kp)
+{
+    for (int off = 1; off < Kp; off <<= 1) v = fmax(v, __shfl_xor(v, off));
+    return v;
+}
+
+template <int SC, typename T, bool SCALE>
+__device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk)
+{
+    constexpr int SM = SC ? SC : MAX_DIM;
+    const int S = SC ? SC : a.dim;
+    const int K = a.K;
+    const int lk = a.log2Kp;
+    const int Kp = 1 << lk;
+    const int tid = threadIdx.x;
+    const int kk0 = tid & (Kp - 1);
+    const bool catOn = kk0 < K;
+    const int kk = catOn ? kk0 : K - 1;
+    const int p = blk * (PRUNE_THREADS >> lk) + (tid >> lk);
+    const size_t nPatPad = (size_t)a.nPatPad;
+    const T *PT = (const T *)a.PT;
+    T *cl = (T *)a.cl;
+
+    double contrib = 0.0;
+    int bad = 0;
+    for (int o = 0; o < a.nOps; ++o) {
+        const OpRec op = a.ops[o];
+        T acc[SM];
+        int e = 0;
+        for (int c = 0; c < op.nKids; ++c) {
+            const KidRec kd = a.kids[op.kidStart + c];
+            const T *Pt = PT + ((size_t)kd.pnode * K + kk) * S * S;
+            T v[SM];
+            bool skip = false;
+            if (kd.kind == 0) {
+                const unsigned code = a.tips[(size_t)kd.idx * nPatPad + p];
+                if (code < (unsigned)S) {
+#pragma unroll
+                    for (int s = 0; s < SM; s++)
+                        if (s < S) v[s] = Pt[code * S + s];
+                } else if (code == TIP_ALL) {
+#pragma unroll
+                    for (int s = 0; s < SM; s++) v[s] = T(1);
+                    skip = (c != 0);
+                } else {
+                    const uint64_t m = a.ambMask[code - S];
+#pragma unroll
+                    for (int s = 0; s < SM; s++) v[s] = T(0);
+                    for (int j = 0; j < S; j++)
+                        if ((m >> j) & 1ull) {
+#pragma unroll
+                            for (int s = 0; s < SM; s++)
+                                if (s < S) v[s] += Pt[j * S + s];
+                        }
+                }
+            } else {
+                const T *x = cl + (((size_t)kd.idx * nPatPad + p) * K + kk) * S;
+                T xv[SM];
+#pragma unroll
+                for (int j = 0; j < SM; j++)
+                    if (j < S) xv[j] = x[j];
+#pragma unroll
+                for (int s = 0; s < SM; s++) v[s] = T(0);
+#pragma unroll
+                for (int j = 0; j < SM; j++) {
+                    if (j < S) {
+#pragma unroll
+                        for (int s = 0; s < SM; s++)
+                            if (s < S) v[s] += xv[j] * Pt[j * S + s];
+                    }
+                }
+                if (SCALE) e += a.scl[(size_t)kd.idx * nPatPad + p];
+            }
+            if (c == 0) {
+#pragma unroll
+                for (int s = 0; s < SM; s++) acc[s] = v[s];
+            } else if (!skip) {
+#pragma unroll
+                for (int s = 0; s < SM; s++) acc[s] *= v[s];
+            }
+        }
+        if (SCALE) {
+            T m = T(0);
+#pragma unroll
+            for (int s = 0; s < SM; s++)
+                if (s < S) m = fmax(m, acc[s]);
+            m = groupMax(m, Kp);
+            if (m < T(SCALE_THRESH) && m > T(0)) {
+                int ex;
+                frexp(m, &ex);
+                const int E = -ex;
+#pragma unroll
+                for (int s = 0; s < SM; s++)
+                    if (s < S) acc[s] = ldexp(acc[s], E);
+                e += E;
+            }
+        }
+        if (!(op.flags & OP_ROOT)) {
+            if (catOn) {
+                T *dst = cl + (((size_t)op.dst * nPatPad + p) * K + kk) * S;
+#pragma unroll
+                for (int s = 0; s < SM; s++)
+                    if (s < S) dst[s] = acc[s];
+            }
+            if (SCALE && kk0 == 0) a.scl[(size_t)op.dst * nPatPad + p] = e;
+        } else {
+            const double *pi = a.rootPi;
+            double Lk = 0.0;
+            if (a.rootLeafRow >= 0) {
+                const unsigned code = a.tips[(size_t)a.rootLeafRow * nPatPad + p];
+                uint64_t m;
+                if (code < (unsigned)S)
+                    m = 1ull << code;
+                else if (code == TIP_ALL)
+                    m = ~0ull;
+                else
+                    m = a.ambMask[code - S];
+#pragma unroll
+                for (int s = 0; s < SM; s++)
+                    if (s < S && ((m >> s) & 1ull)) Lk += pi[s] * (double)acc[s];
+            } else {
+#pragma unroll
+                for (int s = 0; s < SM; s++)
+                    if (s < S) Lk += pi[s] * (double)acc[s];
+            }
+            if (!catOn) Lk = 0.0;
+            for (int off = 1; off < Kp; off <<= 1) Lk += __shfl_xor(Lk, off);
+            if (kk0 == 0) {
+                // Site likelihood as Pf/p4_tree.c:1029-1197.  In FP32 mode
+                // the partials carry 2^-e, so the variable-site part is
+                // unscaled in FP64 before the invariant part is added; with
+                // no invariant part the log is taken in the scaled domain.
+                const double LN2 = 0.69314718055994530942;
+                double like, lnl;
+                bool ok;
+                if (a.pInvar != 0.0) {
+                    const double varS = Lk * ((1.0 - a.pInvar) / (double)K);
+                    double inv = 0.0;
+                    for (int w = 0; w < a.nWords; w++) {
+                        uint32_t mw = a.invMask[(size_t)w * nPatPad + p];
+                        while (mw) {
+                            const int b = __ffs(mw) - 1;
+                            mw &= mw - 1;
+                            inv += pi[w * 32 + b] * a.pInvar;
+                        }
+                    }
+                    if (SCALE && e != 0 && inv == 0.0) {
+                        ok = varS > 0.0;
+                        lnl = log(varS) - (double)e * LN2;
+                        like = ldexp(varS, -e);
+                    } else {
+                        like = ((SCALE && e != 0) ? ldexp(varS, -e) : varS) + inv;
+                        ok = like > 0.0;
+                        lnl = log(like);
+                    }
+                } else {
+                    const double likeS = (K > 1) ? Lk / (double)K : Lk;
+                    ok = likeS > 0.0;
+                    lnl = log(likeS);
+                    like = likeS;
+                    if (SCALE && e != 0) {
+                        lnl -= (double)e * LN2;
+                        like = ldexp(likeS, -e);
+                    }
+                }
+                if (p < a.nPat) {
+                    if (!ok) bad = 1;
+                    contrib = a.counts[p] * lnl;
+                    if (a.patLikes) a.patLikes[p] = like;
+                }
+            }
+        }
+    }
+
+    // workgroup reduction, then the last workgroup of this item sums all
+    for (int off = 32; off >= 1; off >>= 1) {
+        contrib += __shfl_xor(contrib, off);
+        bad |= __shfl_xor(bad, off);
+    }
+    __shared__ double wsum[PRUNE_THREADS / 64];
+    __shared__ int wbad[PRUNE_THREADS / 64];
+    __shared__ int isLast;
+    const int wave = tid >> 6, lane = tid & 63;
+    if (lane == 0) {
+        wsum[wave] = contrib;
+        wbad[wave] = bad;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        int b = 0;
+        for (int w = 0; w < PRUNE_THREADS / 64; w++) {
+            s += wsum[w];
+            b |= wbad[w];
+        }
+        a.blockSums[blk] = s;
+        a.blockBad[blk] = b;
+        __threadfence();
+        unsigned prev = __hip_atomic_fetch_add(a.counter, 1u, __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        isLast = (prev == (unsigned)(nBlk - 1));
+    }
+    __syncthreads();
+    if (isLast) {
+        __threadfence();
+        double s = 0.0, comp = 0.0;
+        int b = 0;
+        for (int i = tid; i < nBlk; i += PRUNE_THREADS) {
+            const double x = __hip_atomic_load(a.blockSums + i, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+            b |= __hip_atomic_load(a.blockBad + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const double t = s + x;  // Neumaier
+            if (fabs(s) >= fabs(x))
+                comp += (s - t) + x;
+            else
+                comp += (x - t) + s;
+            s = t;
+        }
+        double v = s + comp;
+        for (int off = 32; off >= 1; off >>= 1) {
+            v += __shfl_xor(v, off);
+            b |= __shfl_xor(b, off);
+        }
+        if (lane == 0) {
+            wsum[wave] = v;
+            wbad[wave] = b;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double tot = 0.0;
+            int bb = 0;
+            for (int w = 0; w < PRUNE_THREADS / 64; w++) {
+                tot += wsum[w];
+                bb |= wbad[w];
+            }
+            a.result[0] = bb ? -1.0e99 : tot;
+            a.result[1] = (double)bb;
+            *a.counter = 0u;
+        }
+    }
+}
+
+template <int SC, typename T, bool SCALE>
+__global__ __launch_bounds__(PRUNE_THREADS) void k_prune(PruneArgs a)
+{
+    pruneBody<SC, T, SCALE>(a, blockIdx.x, gridDim.x);
+}
+
+// Batched variant: several (tree, part) items of the same state count and
+// precision in one launch.  blockItem[b] = item index; the item's own first
+// block index is found from its tile offset.
+struct BatchItemHdr {
+    int firstBlock, nBlocks;
+};
+
+template <int SC, typename T, bool SCALE>
+__global__ __launch_bounds__(PRUNE_THREADS) void k_pruneBatch(const PruneArgs *args,
+                                                              const int *blockItem)
+{
+    const int item = blockItem[2 * blockIdx.x];
+    const int local = blockItem[2 * blockIdx.x + 1];
+    const PruneArgs a = args[item];
+    const int nBlk = a.nPatPad / (PRUNE_THREADS >> a.log2Kp);
+    pruneBody<SC, T, SCALE>(a, local, nBlk);
+}
+
+template <int SC>
+static hipError_t launchPruneS(const PruneArgs &a, int precision, int nBlocks, hipStream_t s)
+{
+    if (precision == FP32)
+        hipLaunchKernelGGL((k_prune<SC, float, true>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_prune<SC, double, false>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchPrune(const PruneArgs &a, int precision, int nBlocks, hipStream_t s)
+{
+    switch (a.dim) {
+    case 2: return launchPruneS<2>(a, precision, nBlocks, s);
+    case 4: return launchPruneS<4>(a, precision, nBlocks, s);
+    case 20: return launchPruneS<20>(a, precision, nBlocks, s);
+    default: return launchPruneS<0>(a, precision, nBlocks, s);
+    }
+}
+
+template <int SC>
+static hipError_t launchBatchS(const PruneArgs *d, const int *bi, int nBlocks, int precision,
+                               hipStream_t s)
+{
+    if (precision == FP32)
+        hipLaunchKernelGGL((k_pruneBatch<SC, float, true>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s,
+                           d, bi);
+    else
+        hipLaunchKernelGGL((k_pruneBatch<SC, double, false>), dim3(nBlocks), dim3(PRUNE_THREADS), 0,
+                           s, d, bi);
+    return hipGetLastError();
+}
+
+hipError_t launchPruneBatch(const PruneArgs *dArgs, const int *dBlockItem, int nBlocks, int dim,
+                            int precision, hipStream_t s)
+{
+    switch (dim) {
+    case 2: return launchBatchS<2>(dArgs, dBlockItem, nBlocks, precision, s);
+    case 4: return launchBatchS<4>(dArgs, dBlockItem, nBlocks, precision, s);
+    case 20: return launchBatchS<20>(dArgs, dBlockItem, nBlocks, precision, s);
+    default: return launchBatchS<0>(dArgs, dBlockItem, nBlocks, precision, s);
+    }
+}
+
+}  // namespace p4g

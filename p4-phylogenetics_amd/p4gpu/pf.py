@@ -1,0 +1,445 @@
+"""ctypes shim over libp4gpu.so with the names and argument order of p4's
+C extension ``p4.pf`` (Pf/pfmodule.c, method table 2873-3031), for the
+likelihood / MCMC subset.
+
+Handles are returned as Python ints, as the reference does.  Numpy arrays
+that the reference aliases (comp values, gamma val/freqs/rates,
+bQETneedsReset, partLikes, pre/postOrder, the var bounds) are passed by
+pointer and must stay alive and be mutated in place by the caller.
+
+Where the reference prints and calls exit(1), these functions raise
+P4Error with the library's message.  There is no CPU fallback: if the
+library or a GPU is missing, evaluation calls raise.
+"""
+import ctypes
+import os
+
+import numpy
+
+from .p4exceptions import P4Error
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libp4gpu.so")
+
+_lib = None
+
+c_int = ctypes.c_int
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+c_char_p = ctypes.c_char_p
+P_int = ctypes.POINTER(ctypes.c_int)
+P_double = ctypes.POINTER(ctypes.c_double)
+
+
+def _sig(name, res, *args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+def lib():
+    """Load libp4gpu.so (once).  Raises P4Error if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise P4Error(["libp4gpu.so is not built.",
+                       "Run: python p4-phylogenetics_amd/build.py"])
+    _lib = ctypes.CDLL(LIB_PATH)
+    _declare()
+    return _lib
+
+
+def _declare():
+    V, I, D = c_void_p, c_int, c_double
+    _sig("p4g_lastError", c_char_p)
+    _sig("p4g_deviceAvailable", I)
+    _sig("p4g_setKernelTiming", None, I)
+    _sig("p4g_lastKernelMicroseconds", D, V)
+    _sig("p4g_newData", V, I, I)
+    _sig("p4g_freeData", None, V)
+    _sig("p4g_pokePartInData", I, V, V, I)
+    _sig("p4g_newPart", V, I, I, c_char_p, I, c_char_p, I)
+    _sig("p4g_freePart", None, V)
+    _sig("p4g_pokeEquatesTable", I, V, c_char_p)
+    _sig("p4g_pokeSequences", I, V, c_char_p)
+    _sig("p4g_makePatterns", I, V)
+    _sig("p4g_setGlobalInvarSitesVec", I, V)
+    _sig("p4g_partNPatterns", I, V)
+    _sig("p4g_partNChar", I, V)
+    _sig("p4g_partDim", I, V)
+    _sig("p4g_partGetPatterns", I, V, V, V, V, V, V)
+    _sig("p4g_partSetPatterns", I, V, I, V, V)
+    _sig("p4g_getSiteLikes", I, V, V)
+    _sig("p4g_newModel", V, I, I, I, I, I, V, V, V, V, V, V, V, V, V, V, V, V, V, V, V)
+    _sig("p4g_freeModel", None, V)
+    _sig("p4g_newModelPart", I, V, I, I, I, I, I, I, I, V)
+    _sig("p4g_newComp", I, V, I, I, I, V)
+    _sig("p4g_newRMatrix", I, V, I, I, I, I)
+    _sig("p4g_newGdasrv", V, V, I, I, I, I, V, V, V)
+    _sig("p4g_setRMatrixBigR", I, V, I, I, I, I, D)
+    _sig("p4g_setKappa", I, V, I, I, D)
+    _sig("p4g_gdasrvCalcRates", I, V)
+    _sig("p4g_gdasrvCalcRates_np", I, I, D, V, V)
+    _sig("p4g_setPInvarVal", I, V, I, D)
+    _sig("p4g_setRelRateVal", I, V, I, D)
+    _sig("p4g_resetBQET", I, V, I, I, I)
+    _sig("p4g_newTree", V, I, I, V, V, V, V, V)
+    _sig("p4g_freeTree", None, V)
+    _sig("p4g_newNode", V, I, V, I, I, I)
+    _sig("p4g_freeNode", None, V)
+    _sig("p4g_setNodeRelation", I, V, I, I)
+    _sig("p4g_setTreeRoot", I, V, V)
+    _sig("p4g_setBrLen", I, V, D)
+    _sig("p4g_getBrLen", D, V)
+    _sig("p4g_getTreeLen", D, V)
+    _sig("p4g_setCompNum", I, V, I, I)
+    _sig("p4g_setRMatrixNum", I, V, I, I)
+    _sig("p4g_setGdasrvNum", I, V, I, I)
+    _sig("p4g_setPrecision", I, V, I)
+    _sig("p4g_getPrecision", I, V)
+    _sig("p4g_setPrams", I, V, I)
+    _sig("p4g_treeLogLike", I, V, I, P_double)
+    _sig("p4g_partLogLike", I, V, V, I, I, P_double)
+    _sig("p4g_calculateBigPDecks", I, V)
+    _sig("p4g_calculateBigPDecksPart", I, V, I)
+    _sig("p4g_calculateAllBigPDecksAllParts", I, V)
+    _sig("p4g_setConditionalLikelihoodsOfInternalNodePart", I, V, I)
+    _sig("p4g_setConditionalLikelihoodsOfInternalNode", I, V)
+    _sig("p4g_copyCondLikes", I, V, V, I)
+    _sig("p4g_copyBigPDecks", I, V, V, I)
+    _sig("p4g_copyModelPrams", I, V, V)
+    _sig("p4g_verifyIdentityOfTwoTrees", I, V, V)
+    _sig("p4g_getBigPDeck", I, V, I, V)
+    _sig("p4g_getCondLikes", I, V, I, V)
+    _sig("p4g_newBatch", V, I, V)
+    _sig("p4g_freeBatch", None, V)
+    _sig("p4g_batchLogLike", I, V, V)
+
+
+def _err(what):
+    msg = lib().p4g_lastError().decode(errors="replace")
+    raise P4Error([what, msg or "(no message)"])
+
+
+def _ok(status, what):
+    if status != 0:
+        _err(what)
+
+
+def _ptr(a, dtype):
+    if not isinstance(a, numpy.ndarray) or a.dtype != dtype or not a.flags["C_CONTIGUOUS"]:
+        raise P4Error(f"expected a C-contiguous numpy {numpy.dtype(dtype).name} array, got {type(a)}")
+    return a.ctypes.data
+
+
+def _dptr(a):
+    return _ptr(a, numpy.float64)
+
+
+def _iptr(a):
+    return _ptr(a, numpy.int32)
+
+
+def _h(handle, what):
+    if not handle:
+        _err(what)
+    return handle
+
+
+# ------------------------------------------------------------------ status
+def deviceAvailable():
+    return bool(lib().p4g_deviceAvailable())
+
+
+def setKernelTiming(on):
+    lib().p4g_setKernelTiming(1 if on else 0)
+
+
+def lastKernelMicroseconds(cTree):
+    return lib().p4g_lastKernelMicroseconds(cTree)
+
+
+# -------------------------------------------------------------------- data
+def newData(nTax, nParts):
+    return _h(lib().p4g_newData(nTax, nParts), "newData")
+
+
+def freeData(cData):
+    lib().p4g_freeData(cData)
+
+
+def pokePartInData(cPart, cData, partNum):
+    _ok(lib().p4g_pokePartInData(cPart, cData, partNum), "pokePartInData")
+
+
+def newPart(nTax, nChar, equateSymbols, nEquates, symbols, dim):
+    return _h(lib().p4g_newPart(nTax, nChar, equateSymbols.encode(), nEquates,
+                                symbols.encode(), dim), "newPart")
+
+
+def freePart(cPart):
+    lib().p4g_freePart(cPart)
+
+
+def pokeEquatesTable(cPart, theString):
+    _ok(lib().p4g_pokeEquatesTable(cPart, theString.encode()), "pokeEquatesTable")
+
+
+def pokeSequences(cPart, theString):
+    _ok(lib().p4g_pokeSequences(cPart, theString.encode()), "pokeSequences")
+
+
+def makePatterns(cPart):
+    _ok(lib().p4g_makePatterns(cPart), "makePatterns")
+
+
+def setGlobalInvarSitesVec(cPart):
+    _ok(lib().p4g_setGlobalInvarSitesVec(cPart), "setGlobalInvarSitesVec")
+
+
+def partPatternCount(cPart):
+    return lib().p4g_partNPatterns(cPart)
+
+
+def partGetPatterns(cPart, nTax):
+    """(patterns[nTax][nPat], counts[nPat], seqPosPatternIndex[nChar],
+    invarVec[nPat], invarArray[dim][nPat]) as numpy int32 arrays."""
+    L = lib()
+    nP, nC, dim = L.p4g_partNPatterns(cPart), L.p4g_partNChar(cPart), L.p4g_partDim(cPart)
+    pats = numpy.zeros((nTax, nP), numpy.int32)
+    cnts = numpy.zeros(nP, numpy.int32)
+    spi = numpy.zeros(nC, numpy.int32)
+    iv = numpy.zeros(nP, numpy.int32)
+    ia = numpy.zeros((dim, nP), numpy.int32)
+    _ok(L.p4g_partGetPatterns(cPart, pats.ctypes.data, cnts.ctypes.data, spi.ctypes.data,
+                              iv.ctypes.data, ia.ctypes.data), "partGetPatterns")
+    return pats, cnts, spi, iv, ia
+
+
+def partSetPatterns(cPart, patterns, counts):
+    patterns = numpy.ascontiguousarray(patterns, dtype=numpy.int32)
+    counts = numpy.ascontiguousarray(counts, dtype=numpy.int32)
+    _ok(lib().p4g_partSetPatterns(cPart, int(counts.shape[0]), patterns.ctypes.data,
+                                  counts.ctypes.data), "partSetPatterns")
+
+
+def getSiteLikes(cPart):
+    n = lib().p4g_partNChar(cPart)
+    out = numpy.zeros(n, numpy.float64)
+    _ok(lib().p4g_getSiteLikes(cPart, out.ctypes.data), "getSiteLikes")
+    return list(out)
+
+
+# ------------------------------------------------------------------- model
+def p4_newModel(nParts, doRelRates, relRatesAreFree, nFreePrams, isHet, rMatrixNormalizeTo1,
+                PINVAR_MIN, PINVAR_MAX, KAPPA_MIN, KAPPA_MAX, GAMMA_SHAPE_MIN, GAMMA_SHAPE_MAX,
+                PIVEC_MIN, PIVEC_MAX, RATE_MIN, RATE_MAX, RELRATE_MIN, RELRATE_MAX, BRLEN_MIN,
+                BRLEN_MAX):
+    return _h(lib().p4g_newModel(
+        nParts, doRelRates, relRatesAreFree, int(nFreePrams), isHet, _iptr(rMatrixNormalizeTo1),
+        _dptr(PINVAR_MIN), _dptr(PINVAR_MAX), _dptr(KAPPA_MIN), _dptr(KAPPA_MAX),
+        _dptr(GAMMA_SHAPE_MIN), _dptr(GAMMA_SHAPE_MAX), _dptr(PIVEC_MIN), _dptr(PIVEC_MAX),
+        _dptr(RATE_MIN), _dptr(RATE_MAX), _dptr(RELRATE_MIN), _dptr(RELRATE_MAX),
+        _dptr(BRLEN_MIN), _dptr(BRLEN_MAX)), "p4_newModel")
+
+
+def p4_freeModel(cModel):
+    lib().p4g_freeModel(cModel)
+
+
+def p4_newModelPart(cModel, pNum, dim, nComps, nRMatrices, nGdasrvs, nCat, pInvarFree,
+                    bQETneedsReset):
+    _ok(lib().p4g_newModelPart(cModel, pNum, dim, nComps, nRMatrices, nGdasrvs, nCat,
+                               pInvarFree, _iptr(bQETneedsReset)), "p4_newModelPart")
+
+
+def p4_newComp(cModel, pNum, mNum, free, val):
+    _ok(lib().p4g_newComp(cModel, pNum, mNum, free, _dptr(val)), "p4_newComp")
+
+
+def p4_newRMatrix(cModel, pNum, mNum, free, spec):
+    _ok(lib().p4g_newRMatrix(cModel, pNum, mNum, free, spec), "p4_newRMatrix")
+
+
+def p4_newGdasrv(cModel, pNum, mNum, nCat, free, val, freqs, rates):
+    return _h(lib().p4g_newGdasrv(cModel, pNum, mNum, nCat, free, _dptr(val), _dptr(freqs),
+                                  _dptr(rates)), "p4_newGdasrv")
+
+
+def p4_setRMatrixBigR(cModel, pNum, mNum, i, j, val):
+    _ok(lib().p4g_setRMatrixBigR(cModel, pNum, mNum, i, j, float(val)), "p4_setRMatrixBigR")
+
+
+def p4_setKappa(cModel, pNum, mNum, val):
+    _ok(lib().p4g_setKappa(cModel, pNum, mNum, float(val)), "p4_setKappa")
+
+
+def gdasrvCalcRates(cGdasrv):
+    _ok(lib().p4g_gdasrvCalcRates(cGdasrv), "gdasrvCalcRates")
+
+
+def gdasrvCalcRates_np(nCat, val, freqs, rates):
+    _ok(lib().p4g_gdasrvCalcRates_np(nCat, float(val), _dptr(freqs), _dptr(rates)),
+        "gdasrvCalcRates_np")
+
+
+def p4_setPInvarVal(cModel, pNum, val):
+    _ok(lib().p4g_setPInvarVal(cModel, pNum, float(val)), "p4_setPInvarVal")
+
+
+def p4_setRelRateVal(cModel, pNum, val):
+    _ok(lib().p4g_setRelRateVal(cModel, pNum, float(val)), "p4_setRelRateVal")
+
+
+def p4_resetBQET(cModel, pNum, compNum, rMatrixNum):
+    _ok(lib().p4g_resetBQET(cModel, pNum, compNum, rMatrixNum), "p4_resetBQET")
+
+
+# -------------------------------------------------------------------- tree
+def p4_newTree(nNodes, nLeaves, preOrder, postOrder, passLimit, partLikes, cData, cModel):
+    """Same argument list as the reference (Pf/pfmodule.c:1383-1406);
+    passLimit (newtAndBrentPowellOptPassLimit) is accepted and unused."""
+    return _h(lib().p4g_newTree(nNodes, nLeaves, _iptr(preOrder), _iptr(postOrder),
+                                _dptr(partLikes), cData, cModel), "p4_newTree")
+
+
+def p4_freeTree(cTree):
+    lib().p4g_freeTree(cTree)
+
+
+def p4_newNode(nodeNum, cTree, seqNum, isLeaf, inTree):
+    return _h(lib().p4g_newNode(nodeNum, cTree, seqNum, isLeaf, inTree), "p4_newNode")
+
+
+def p4_freeNode(cNode):
+    lib().p4g_freeNode(cNode)
+
+
+def p4_setNodeRelation(cNode, relation, relNum):
+    _ok(lib().p4g_setNodeRelation(cNode, relation, relNum), "p4_setNodeRelation")
+
+
+def p4_setTreeRoot(cTree, cNode):
+    _ok(lib().p4g_setTreeRoot(cTree, cNode), "p4_setTreeRoot")
+
+
+def p4_setBrLen(cNode, brLen):
+    _ok(lib().p4g_setBrLen(cNode, float(brLen)), "p4_setBrLen")
+
+
+def p4_getBrLen(cNode):
+    return lib().p4g_getBrLen(cNode)
+
+
+def p4_getTreeLen(cTree):
+    return lib().p4g_getTreeLen(cTree)
+
+
+def p4_setCompNum(cNode, pNum, val):
+    _ok(lib().p4g_setCompNum(cNode, pNum, val), "p4_setCompNum")
+
+
+def p4_setRMatrixNum(cNode, pNum, val):
+    _ok(lib().p4g_setRMatrixNum(cNode, pNum, val), "p4_setRMatrixNum")
+
+
+def p4_setGdasrvNum(cNode, pNum, val):
+    _ok(lib().p4g_setGdasrvNum(cNode, pNum, val), "p4_setGdasrvNum")
+
+
+def p4_setPrecision(cTree, fp32):
+    _ok(lib().p4g_setPrecision(cTree, 1 if fp32 else 0), "p4_setPrecision")
+
+
+# -------------------------------------------------------------- evaluation
+def p4_setPrams(cTree, pNum):
+    _ok(lib().p4g_setPrams(cTree, pNum), "p4_setPrams")
+
+
+def p4_treeLogLike(cTree, getSiteLikes):
+    out = c_double(0.0)
+    _ok(lib().p4g_treeLogLike(cTree, getSiteLikes, ctypes.byref(out)), "p4_treeLogLike")
+    return out.value
+
+
+def p4_partLogLike(cTree, cPart, pNum, getSiteLikes):
+    out = c_double(0.0)
+    _ok(lib().p4g_partLogLike(cTree, cPart, pNum, getSiteLikes, ctypes.byref(out)),
+        "p4_partLogLike")
+    return out.value
+
+
+def p4_calculateBigPDecks(cNode):
+    _ok(lib().p4g_calculateBigPDecks(cNode), "p4_calculateBigPDecks")
+
+
+def p4_calculateBigPDecksPart(cNode, pNum):
+    _ok(lib().p4g_calculateBigPDecksPart(cNode, pNum), "p4_calculateBigPDecksPart")
+
+
+def p4_calculateAllBigPDecksAllParts(cTree):
+    _ok(lib().p4g_calculateAllBigPDecksAllParts(cTree), "p4_calculateAllBigPDecksAllParts")
+
+
+def p4_setConditionalLikelihoodsOfInternalNodePart(cNode, pNum):
+    _ok(lib().p4g_setConditionalLikelihoodsOfInternalNodePart(cNode, pNum),
+        "p4_setConditionalLikelihoodsOfInternalNodePart")
+
+
+def p4_setConditionalLikelihoodsOfInternalNode(cNode):
+    _ok(lib().p4g_setConditionalLikelihoodsOfInternalNode(cNode),
+        "p4_setConditionalLikelihoodsOfInternalNode")
+
+
+def p4_copyCondLikes(cTreeA, cTreeB, doAll):
+    _ok(lib().p4g_copyCondLikes(cTreeA, cTreeB, doAll), "p4_copyCondLikes")
+
+
+def p4_copyBigPDecks(cTreeA, cTreeB, doAll):
+    _ok(lib().p4g_copyBigPDecks(cTreeA, cTreeB, doAll), "p4_copyBigPDecks")
+
+
+def p4_copyModelPrams(cTreeA, cTreeB):
+    _ok(lib().p4g_copyModelPrams(cTreeA, cTreeB), "p4_copyModelPrams")
+
+
+def p4_verifyIdentityOfTwoTrees(cTreeA, cTreeB):
+    return lib().p4g_verifyIdentityOfTwoTrees(cTreeA, cTreeB)
+
+
+def p4_getBigPDeck(cNode, pNum, nCat, dim):
+    out = numpy.zeros((nCat, dim, dim), numpy.float64)
+    _ok(lib().p4g_getBigPDeck(cNode, pNum, out.ctypes.data), "p4_getBigPDeck")
+    return out
+
+
+def p4_getCondLikes(cNode, pNum, nCat, dim, nPatterns):
+    out = numpy.zeros((nCat, dim, nPatterns), numpy.float64)
+    _ok(lib().p4g_getCondLikes(cNode, pNum, out.ctypes.data), "p4_getCondLikes")
+    return out
+
+
+# ----------------------------------------------------------------- batches
+class Batch:
+    """All pending partials and site sums of several trees in one launch
+    (per state count and precision)."""
+
+    def __init__(self, cTrees):
+        self.n = len(cTrees)
+        arr = (c_void_p * self.n)(*cTrees)
+        self._arr = arr
+        self.c = _h(lib().p4g_newBatch(self.n, ctypes.cast(arr, c_void_p)), "newBatch")
+        self.out = numpy.zeros(self.n, numpy.float64)
+
+    def logLikes(self):
+        _ok(lib().p4g_batchLogLike(self.c, self.out.ctypes.data), "batchLogLike")
+        return self.out.copy()
+
+    def __del__(self):
+        try:
+            if self.c and _lib is not None:
+                _lib.p4g_freeBatch(self.c)
+        except Exception:
+            pass
