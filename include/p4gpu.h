@@ -216,8 +216,11 @@ int p4g_getCondLikes(const p4g_node *n, int pNum, double *out);
 p4g_batch *p4g_newBatch(int nTrees, p4g_tree **trees);
 void p4g_freeBatch(p4g_batch *b);
 int p4g_batchLogLike(p4g_batch *b, double *lnL);
-/* Mean device time in microseconds of the last n launches of the pruning
- * kernel of tree t (HIP events on the tree's stream); -1 if unknown. */
+/* Kernel timing (no counterpart in the reference): when on, every pruning
+ * launch is bracketed by HIP events on the library's stream, and
+ * p4g_lastKernelMicroseconds returns the device time of the last
+ * evaluation's pruning launches of tree t (summed over parts), -1 if none. */
+void p4g_setKernelTiming(int on);
 double p4g_lastKernelMicroseconds(const p4g_tree *t);
 
 #ifdef __cplusplus

@@ -240,6 +240,7 @@ def flat_from_tree(tree, pNum=0, patterns=None, counts=None, own_compression=Fal
     (a) the arguments, (b) the oracle's own compression of the alignment's
     sequences (own_compression=True), or (c) the Part's patterns."""
     from p4gpu.synth import bigR_from_vals  # R-vector layout of p4/model.py:185-190
+    tree.modelSanityCheck(resetEmpiricalComps=False)  # fills node model numbers
     mp = tree.model.parts[pNum]
     dp = tree.data.parts[pNum]
     dim = mp.dim
@@ -286,6 +287,8 @@ def part_loglike(f, site_likes=False):
     pl = numpy.zeros(f.patterns.shape[1]) if site_likes else None
     v = lib().or_tree_loglike_part(ctypes.byref(f.tr), ctypes.byref(f.mp), ctypes.byref(f.dp),
                                    _p(pl))
+    if v <= -2.0e99:
+        raise ValueError("oracle rejected the input (model numbers, comps or allocation)")
     return (v, pl) if site_likes else v
 
 

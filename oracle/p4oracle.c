@@ -359,6 +359,11 @@ int or_set_prams_part(const or_tree *tr, const or_modelpart *mp, double *P)
         if (n == tr->root) continue;
         c = tr->compNums[n];
         r = tr->rMatrixNums[n];
+        if (c < 0 || c >= mp->nComps || r < 0 || r >= mp->nRMatrices ||
+            (mp->nGdasrvs && (tr->gdasrvNums[n] < 0 || tr->gdasrvNums[n] >= mp->nGdasrvs))) {
+            ret = 6; /* node model numbers out of range */
+            goto out;
+        }
         if (!done[c * mp->nRMatrices + r]) {
             int slot = c * mp->nRMatrices + r;
             if (or_bigq(dim, mp->bigR + (size_t)r * dim * dim, mp->comps + (size_t)c * dim,

@@ -122,7 +122,6 @@ Model::~Model()
 
 static void freeTreePart(TreePart &t)
 {
-    devFree(t.eigIn);
     devFree(t.eigV);
     devFree(t.eigVi);
     devFree(t.eigLam);
@@ -138,8 +137,14 @@ static void freeTreePart(TreePart &t)
     devFree(t.patLikes);
     devFree(t.prog);
     devFree(t.rootPi);
-    devFree(t.dStage);
-    hostFree(t.hStage);
+    devFree(t.dEigIn);
+    devFree(t.dPack);
+    hostFree(t.hEigIn);
+    hostFree(t.hEigList);
+    hostFree(t.hT);
+    hostFree(t.hEs);
+    hostFree(t.hPack);
+    hostFree(t.hRootPi);
     hostFree(t.hProg);
     hostFree(t.hResult);
     if (t.ev0) (void)hipEventDestroy(t.ev0);
@@ -330,8 +335,7 @@ static bool allocTreePart(Tree *t, int pNum)
     tp.maxBlocks = tp.nPatPad / PB;
     const size_t elt = t->precision == FP32 ? sizeof(float) : sizeof(double);
     tp.clBytes = (size_t)tp.nPatPad * K * S * elt;
-    if (!devAlloc(&tp.eigIn, (size_t)tp.nEig * (S + S * S), "hipMalloc eigIn") ||
-        !devAlloc(&tp.eigV, (size_t)tp.nEig * S * S, "hipMalloc eigV") ||
+    if (!devAlloc(&tp.eigV, (size_t)tp.nEig * S * S, "hipMalloc eigV") ||
         !devAlloc(&tp.eigVi, (size_t)tp.nEig * S * S, "hipMalloc eigVi") ||
         !devAlloc(&tp.eigLam, (size_t)tp.nEig * S, "hipMalloc eigLam") ||
         !devAlloc(&tp.PT64, nPS, "hipMalloc P64") || !devAlloc(&tp.PT32, nPS, "hipMalloc P32") ||
@@ -350,15 +354,25 @@ static bool allocTreePart(Tree *t, int pNum)
         if (!hostAlloc(&tp.hProg, tp.progCap, "hipHostMalloc program")) return false;
         tp.hProgCap = tp.progCap;
     }
-    if (!tp.hStage) {
-        tp.hStageCap = (size_t)tp.nEig * (S + S * S) + (size_t)t->nNodes * (K + 2) + MAX_DIM + 64;
-        if (!hostAlloc(&tp.hStage, tp.hStageCap, "hipHostMalloc stage")) return false;
-        if (!devAlloc(&tp.dStage, tp.hStageCap, "hipMalloc stage")) return false;
-        if (!hostAlloc(&tp.hResult, 2, "hipHostMalloc result")) return false;
+    if (!tp.hEigIn) {
+        const size_t eigWords = (size_t)tp.nEig * (S + S * S);
+        tp.packBytes = (size_t)t->nNodes * (K * sizeof(double) + 2 * sizeof(int));
+        if (!hostAlloc(&tp.hEigIn, eigWords, "hipHostMalloc eigen inputs") ||
+            !hostAlloc(&tp.hEigList, tp.nEig, "hipHostMalloc eigen list") ||
+            !devAlloc(&tp.dEigIn, eigWords + tp.nEig, "hipMalloc eigen inputs") ||
+            !hostAlloc(&tp.hT, (size_t)t->nNodes * K, "hipHostMalloc branch lengths") ||
+            !hostAlloc(&tp.hEs, t->nNodes, "hipHostMalloc eigen slots") ||
+            !hostAlloc(&tp.hPack, tp.packBytes, "hipHostMalloc P items") ||
+            !devAlloc(&tp.dPack, tp.packBytes, "hipMalloc P items") ||
+            !hostAlloc(&tp.hRootPi, MAX_DIM, "hipHostMalloc root comp") ||
+            !hostAlloc(&tp.hResult, 2, "hipHostMalloc result"))
+            return false;
         if (!hipOk(hipEventCreate(&tp.ev0), "hipEventCreate") ||
             !hipOk(hipEventCreate(&tp.ev1), "hipEventCreate"))
             return false;
     }
+    tp.eigPending.assign(tp.nEig, 0);
+    tp.pPending.assign(t->nNodes, 0);
     if (!hipOk(hipMemsetAsync(tp.counter, 0, sizeof(unsigned), stream()), "zero counter")) return false;
     tp.dirty.assign(t->nNodes, 1);
     tp.pValid.assign(t->nNodes, 0);
@@ -420,66 +434,91 @@ static bool checkNodeModel(const Tree *t, const ModelPart *mp, const Node *nd, i
     return true;
 }
 
-// Compute the eigensystems of the listed (comp, rMatrix) slots on the GPU.
-static bool runEig(Tree *t, int pNum, const std::vector<int> &slots)
+// Snapshot the inputs of eigensystem `slot` (pi of its comp, R of its
+// rMatrix) and mark it for the next flush.
+static void markEig(Tree *t, int pNum, int slot)
 {
-    if (slots.empty()) return true;
     TreePart &tp = t->tp[pNum];
     ModelPart *mp = t->model->parts[pNum];
     const int S = mp->dim;
-    double *h = tp.hStage;
-    int *hs = reinterpret_cast<int *>(h + (size_t)tp.nEig * (S + S * S));
-    // Reuse the staging area: inputs for all slots, then the slot list.
-    for (int slot : slots) {
-        const int c = slot / mp->nRMatrices, r = slot % mp->nRMatrices;
-        double *dst = h + (size_t)slot * (S + S * S);
-        std::memcpy(dst, mp->comps[c].val, sizeof(double) * S);
-        std::memcpy(dst + S, mp->rMatrices[r].bigR.data(), sizeof(double) * S * S);
-    }
-    for (size_t i = 0; i < slots.size(); i++) hs[i] = slots[i];
-    hipStream_t s = stream();
-    const size_t inWords = (size_t)tp.nEig * (S + S * S);
-    const size_t bytes = sizeof(double) * inWords + sizeof(int) * slots.size();
-    if (!hipOk(hipMemcpyAsync(tp.dStage, h, bytes, hipMemcpyHostToDevice, s), "stage eigen inputs"))
-        return false;
-    const int *dSlots = reinterpret_cast<const int *>(tp.dStage + inWords);
-    EigArgs a{S, (int)slots.size(), dSlots, tp.dStage, tp.eigV, tp.eigVi, tp.eigLam};
-    if (!hipOk(launchEig(a, s), "launch eigensystem kernel")) return false;
-    // the staging buffers are reused by the next host step: wait here
-    return hipOk(hipStreamSynchronize(s), "eigensystem kernel");
+    const int c = slot / mp->nRMatrices, r = slot % mp->nRMatrices;
+    double *dst = tp.hEigIn + (size_t)slot * (S + S * S);
+    std::memcpy(dst, mp->comps[c].val, sizeof(double) * S);
+    std::memcpy(dst + S, mp->rMatrices[r].bigR.data(), sizeof(double) * S * S);
+    tp.eigPending[slot] = 1;
 }
 
-static bool runPmat(Tree *t, int pNum, const std::vector<int> &nodes)
+// Snapshot the effective branch lengths of node n (Pf/p4_node.c:321-346)
+// and mark its P matrices for the next flush.
+static void markP(Tree *t, int pNum, const Node *nd)
 {
-    if (nodes.empty()) return true;
     TreePart &tp = t->tp[pNum];
     ModelPart *mp = t->model->parts[pNum];
-    const int K = mp->nCat;
-    const int nI = (int)nodes.size();
-    double *ht = tp.hStage;
-    int *hn = reinterpret_cast<int *>(ht + (size_t)nI * K);
-    int *he = hn + nI;
-    for (int i = 0; i < nI; i++) {
-        const Node *nd = t->nodes[nodes[i]];
-        for (int k = 0; k < K; k++) ht[(size_t)i * K + k] = branchT(t, mp, nd, pNum, k);
-        hn[i] = nodes[i];
-        he[i] = nd->compNums[pNum] * mp->nRMatrices + nd->rMatrixNums[pNum];
-    }
-    const size_t bytes = sizeof(double) * nI * K + sizeof(int) * 2 * nI;
+    const int K = mp->nCat, n = nd->nodeNum;
+    for (int k = 0; k < K; k++) tp.hT[(size_t)n * K + k] = branchT(t, mp, nd, pNum, k);
+    tp.hEs[n] = nd->compNums[pNum] * mp->nRMatrices + nd->rMatrixNums[pNum];
+    tp.pPending[n] = 1;
+    tp.pValid[n] = 1;
+}
+
+// Enqueue the pending eigensystems, then the pending P matrices, of one
+// part (one launch each) on the library stream.
+static bool flushParams(Tree *t, int pNum)
+{
+    TreePart &tp = t->tp[pNum];
+    ModelPart *mp = t->model->parts[pNum];
+    const int S = mp->dim, K = mp->nCat;
     hipStream_t s = stream();
-    if (bytes > tp.hStageCap * sizeof(double)) {
-        setError("internal: staging buffer too small");
-        return false;
+    int nE = 0;
+    for (int i = 0; i < tp.nEig; i++)
+        if (tp.eigPending[i]) {
+            tp.hEigList[nE++] = i;
+            tp.eigPending[i] = 0;
+        }
+    if (nE) {
+        const size_t words = (size_t)tp.nEig * (S + S * S);
+        if (!hipOk(hipMemcpyAsync(tp.dEigIn, tp.hEigIn, sizeof(double) * words, hipMemcpyHostToDevice, s),
+                   "stage eigen inputs") ||
+            !hipOk(hipMemcpyAsync(tp.dEigIn + words, tp.hEigList, sizeof(int) * nE, hipMemcpyHostToDevice, s),
+                   "stage eigen list"))
+            return false;
+        EigArgs a{S, nE, reinterpret_cast<const int *>(tp.dEigIn + words), tp.dEigIn, tp.eigV, tp.eigVi,
+                  tp.eigLam};
+        if (!hipOk(launchEig(a, s), "launch eigensystem kernel")) return false;
     }
-    char *d = reinterpret_cast<char *>(tp.dStage);
-    if (!hipOk(hipMemcpyAsync(d, ht, bytes, hipMemcpyHostToDevice, s), "stage P inputs")) return false;
-    PmatArgs a{mp->dim, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
-               reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
-               reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
-    if (!hipOk(launchPmat(a, s), "launch P kernel")) return false;
-    if (!hipOk(hipStreamSynchronize(s), "P kernel")) return false;
-    for (int n : nodes) tp.pValid[n] = 1;
+    int nI = 0;
+    for (int n = 0; n < t->nNodes; n++) nI += tp.pPending[n] ? 1 : 0;
+    if (nI) {
+        double *ht = reinterpret_cast<double *>(tp.hPack);
+        int *hn = reinterpret_cast<int *>(ht + (size_t)nI * K);
+        int *he = hn + nI;
+        int i = 0;
+        for (int n = 0; n < t->nNodes; n++) {
+            if (!tp.pPending[n]) continue;
+            for (int k = 0; k < K; k++) ht[(size_t)i * K + k] = tp.hT[(size_t)n * K + k];
+            hn[i] = n;
+            he[i] = tp.hEs[n];
+            tp.pPending[n] = 0;
+            i++;
+        }
+        const size_t bytes = sizeof(double) * nI * K + sizeof(int) * 2 * nI;
+        if (!hipOk(hipMemcpyAsync(tp.dPack, tp.hPack, bytes, hipMemcpyHostToDevice, s), "stage P inputs"))
+            return false;
+        const char *d = tp.dPack;
+        PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
+                   reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
+                   reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
+        if (!hipOk(launchPmat(a, s), "launch P kernel")) return false;
+    }
     return true;
+}
+
+// Flush every part and wait (for inspection and copy entry points).
+static bool flushAll(Tree *t)
+{
+    for (int p = 0; p < t->nParts; p++)
+        if (t->tp[p].alloc && !flushParams(t, p)) return false;
+    return hipOk(hipStreamSynchronize(stream()), "parameter kernels");
 }
 
 // Pf/p4_tree.c:218-541 for one part.
@@ -536,10 +575,11 @@ static bool setPramsPart(Tree *t, int pNum)
         }
         nodes.push_back(n);
     }
-    if (!runEig(t, pNum, slots)) return false;
+    for (int slot : slots) markEig(t, pNum, slot);
     if (mp->bQETneedsReset)
         for (int slot : slots) mp->bQETneedsReset[slot] = 0;
-    return runPmat(t, pNum, nodes);
+    for (int n : nodes) markP(t, pNum, t->nodes[n]);
+    return true;
 }
 
 // P of one node (Pf/p4_node.c:296-346), recomputing a stale eigensystem
@@ -551,10 +591,11 @@ static bool bigPDecksPart(Tree *t, Node *nd, int pNum)
     if (!checkNodeModel(t, mp, nd, pNum)) return false;
     const int slot = nd->compNums[pNum] * mp->nRMatrices + nd->rMatrixNums[pNum];
     if (mp->bQETneedsReset && mp->bQETneedsReset[slot]) {
-        if (!runEig(t, pNum, {slot})) return false;
+        markEig(t, pNum, slot);
         mp->bQETneedsReset[slot] = 0;
     }
-    return runPmat(t, pNum, {nd->nodeNum});
+    markP(t, pNum, nd);
+    return true;
 }
 
 // Build the node program for part pNum: all internal nodes in postorder
@@ -628,7 +669,48 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
         setError("postOrder does not end at the root");
         return false;
     }
-    const size_t words = ops.size() * 4 + kids.size() * 4;
+    // Group ops by height above the nodes already computed: ops of equal
+    // height are independent, so binary ones are packed G to a group.
+    const int G = pruneGroupWidth(t->model->parts[pNum]->dim);
+    const int nOps = (int)ops.size();
+    std::vector<int> opOfSlot(tp.nSlots, -1), height(nOps, 0);
+    for (int i = 0; i < nOps; i++) {
+        int h = 0;
+        for (int c = 0; c < ops[i].nKids; c++) {
+            const KidRec &k = kids[ops[i].kidStart + c];
+            if (k.kind == 1 && opOfSlot[k.idx] >= 0) h = std::max(h, height[opOfSlot[k.idx]] + 1);
+        }
+        height[i] = h;
+        if (!(ops[i].flags & OP_ROOT)) opOfSlot[ops[i].dst] = i;
+    }
+    std::vector<int> order(nOps - 1);
+    for (int i = 0; i < nOps - 1; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return height[x] < height[y]; });
+    order.push_back(nOps - 1);  // the root last
+    std::vector<OpRec> ops2;
+    std::vector<KidRec> kids2;
+    std::vector<GroupRec> groups;
+    for (size_t q = 0; q < order.size();) {
+        const int i = order[q];
+        const bool binary = ops[i].nKids == 2 && !(ops[i].flags & OP_ROOT);
+        GroupRec gr{binary ? 0 : 1, 0, (int)ops2.size(), 0};
+        size_t r = q;
+        while (r < order.size()) {
+            const int j = order[r];
+            const bool bj = ops[j].nKids == 2 && !(ops[j].flags & OP_ROOT);
+            if (gr.nOps > 0 && (!binary || !bj || height[j] != height[i] || gr.nOps == G)) break;
+            OpRec o = ops[j];
+            o.kidStart = (int)kids2.size();
+            for (int c = 0; c < ops[j].nKids; c++) kids2.push_back(kids[ops[j].kidStart + c]);
+            ops2.push_back(o);
+            gr.nOps++;
+            r++;
+            if (!binary) break;
+        }
+        groups.push_back(gr);
+        q = r;
+    }
+    const size_t words = groups.size() * 4 + ops2.size() * 4 + kids2.size() * 4;
     if ((int)words > tp.hProgCap) {
         hostFree(tp.hProg);
         devFree(tp.prog);
@@ -637,10 +719,13 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
             !hostAlloc(&tp.hProg, tp.hProgCap, "hipHostMalloc program"))
             return false;
     }
-    std::memcpy(tp.hProg, ops.data(), ops.size() * sizeof(OpRec));
-    std::memcpy(tp.hProg + ops.size() * 4, kids.data(), kids.size() * sizeof(KidRec));
-    *nOpsOut = (int)ops.size();
-    *nKidsOut = (int)kids.size();
+    std::memcpy(tp.hProg, groups.data(), groups.size() * sizeof(GroupRec));
+    std::memcpy(tp.hProg + groups.size() * 4, ops2.data(), ops2.size() * sizeof(OpRec));
+    std::memcpy(tp.hProg + (groups.size() + ops2.size()) * 4, kids2.data(), kids2.size() * sizeof(KidRec));
+    tp.nGroups = (int)groups.size();
+    tp.nKids = (int)kids2.size();
+    *nOpsOut = (int)ops2.size();
+    *nKidsOut = (int)kids2.size();
     return true;
 }
 
@@ -650,9 +735,12 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     ModelPart *mp = t->model->parts[pNum];
     Part *dp = t->data->parts[pNum];
     const Node *root = t->nodes[t->root];
-    a.ops = reinterpret_cast<const OpRec *>(tp.prog);
-    a.kids = reinterpret_cast<const KidRec *>(tp.prog + nOps * 4);
+    a.groups = reinterpret_cast<const GroupRec *>(tp.prog);
+    a.ops = reinterpret_cast<const OpRec *>(tp.prog + tp.nGroups * 4);
+    a.kids = reinterpret_cast<const KidRec *>(tp.prog + (tp.nGroups + nOps) * 4);
+    a.nGroups = tp.nGroups;
     a.nOps = nOps;
+    a.nKids = tp.nKids;
     a.dim = mp->dim;
     a.K = mp->nCat;
     a.log2Kp = log2ceil(mp->nCat);
@@ -689,10 +777,11 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps)
         setError("root comp out of range");
         return false;
     }
-    double *hpi = tp.hStage + (tp.hStageCap - MAX_DIM);
+    double *hpi = tp.hRootPi;
     std::memcpy(hpi, mp->comps[rc].val, sizeof(double) * mp->dim);
     hipStream_t s = stream();
-    return hipOk(hipMemcpyAsync(tp.prog, tp.hProg, sizeof(int) * 4 * (*nOps + nKids),
+    if (!flushParams(t, pNum)) return false;
+    return hipOk(hipMemcpyAsync(tp.prog, tp.hProg, sizeof(int) * 4 * (tp.nGroups + *nOps + nKids),
                                 hipMemcpyHostToDevice, s),
                  "stage program") &&
            hipOk(hipMemcpyAsync(tp.rootPi, hpi, sizeof(double) * mp->dim, hipMemcpyHostToDevice, s),
@@ -949,10 +1038,11 @@ int p4g_partSetPatterns(p4g_part *p, int nPatterns, const int *patterns, const i
     p->nPatterns = nPatterns;
     p->patterns.assign(patterns, patterns + (size_t)p->nTax * nPatterns);
     p->patternCounts.assign(counts, counts + nPatterns);
+    // sites are taken in pattern order, each pattern repeated count times
     p->seqPosPatternIndex.clear();
-    int n = 0;
-    for (int k = 0; k < nPatterns; k++) n += counts[k];
-    p->nChar = n;
+    for (int k = 0; k < nPatterns; k++)
+        for (int c = 0; c < counts[k]; c++) p->seqPosPatternIndex.push_back(k);
+    p->nChar = (int)p->seqPosPatternIndex.size();
     p->invarSet = false;
     p->dev.valid = false;
     return checkCodes(p) ? 0 : 1;
@@ -1394,12 +1484,11 @@ int p4g_calculateAllBigPDecksAllParts(p4g_tree *t)
             if (!checkNodeModel(t, mp, nd, p)) return 1;
             const int slot = nd->compNums[p] * mp->nRMatrices + nd->rMatrixNums[p];
             if (mp->bQETneedsReset && mp->bQETneedsReset[slot]) {
-                if (!runEig(t, p, {slot})) return 1;
+                markEig(t, p, slot);
                 mp->bQETneedsReset[slot] = 0;
             }
-            nodes.push_back(n);
+            markP(t, p, nd);
         }
-        if (!runPmat(t, p, nodes)) return 1;
     }
     return 0;
 }
@@ -1486,6 +1575,7 @@ int p4g_copyBigPDecks(p4g_tree *a, p4g_tree *b, int doAll)
         return 1;
     }
     if (!a || !b || !sameShape(a, b) || !treeReady(a) || !treeReady(b)) return 1;
+    if (!flushAll(a) || !flushAll(b)) return 1;
     hipStream_t s = stream();
     for (int p = 0; p < a->nParts; p++) {
         TreePart &A = a->tp[p], &B = b->tp[p];
@@ -1503,6 +1593,7 @@ int p4g_copyModelPrams(p4g_tree *a, p4g_tree *b)
 {
     clearError();
     if (!a || !b || a->model->nParts != b->model->nParts) return 1;
+    if ((a->tp[0].alloc && !flushAll(a)) || (b->tp[0].alloc && !flushAll(b))) return 1;
     hipStream_t s = stream();
     for (int p = 0; p < a->model->nParts; p++) {
         ModelPart *A = a->model->parts[p], *B = b->model->parts[p];
@@ -1538,7 +1629,7 @@ int p4g_verifyIdentityOfTwoTrees(p4g_tree *a, p4g_tree *b)
 {
     clearError();
     if (!a || !b || !sameShape(a, b) || !treeReady(a) || !treeReady(b)) return 1;
-    if (!hipOk(hipStreamSynchronize(stream()), "sync")) return 1;
+    if (!flushAll(a) || !flushAll(b)) return 1;
     for (int p = 0; p < a->nParts; p++) {
         TreePart &A = a->tp[p], &B = b->tp[p];
         const size_t n = (size_t)a->nNodes * A.K * A.dim * A.dim;
@@ -1587,7 +1678,7 @@ int p4g_getBigPDeck(const p4g_node *n, int pNum, double *out)
     }
     const int K = tp.K, S = tp.dim;
     std::vector<double> h((size_t)K * S * S);
-    if (!hipOk(hipStreamSynchronize(stream()), "sync") ||
+    if (!flushAll(n->tree) ||
         !hipOk(hipMemcpy(h.data(), tp.PT64 + (size_t)n->nodeNum * K * S * S, h.size() * 8, hipMemcpyDeviceToHost),
                "copy P"))
         return 1;
@@ -1726,6 +1817,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     int j = 0, blk = 0;
     std::vector<std::pair<int, int>> launches;  // (first block, count) per group
     std::vector<int> groupDim, groupPrec;
+    std::vector<size_t> groupLds;
     while (j < nItems) {
         const int k0 = key(order[j]);
         const int first = blk;
@@ -1743,6 +1835,16 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         launches.push_back({first, blk - first});
         groupDim.push_back(items[order[j]].t->model->parts[items[order[j]].p]->dim);
         groupPrec.push_back(items[order[j]].t->precision);
+        // P in LDS for the whole launch only if every item's P rows fit
+        size_t lds = 0;
+        bool fits = true;
+        for (int q = j; q < jj; q++) {
+            const PruneArgs &pa = b->hArgs[q];
+            const size_t l = pruneLdsBytes(pa.nKids, pa.K, pa.dim, groupPrec.back());
+            if (!l) fits = false;
+            lds = std::max(lds, l);
+        }
+        groupLds.push_back(fits ? lds : 0);
         j = jj;
     }
     if (!hipOk(hipMemcpyAsync(b->dBlockItem, b->hBlockItem, sizeof(int) * 2 * blk, hipMemcpyHostToDevice, s),
@@ -1750,7 +1852,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         return 1;
     for (size_t g = 0; g < launches.size(); g++)
         if (!hipOk(launchPruneBatch(b->dArgs, b->dBlockItem + 2 * launches[g].first, launches[g].second,
-                                    groupDim[g], groupPrec[g], s),
+                                    groupDim[g], groupPrec[g], groupLds[g], s),
                    "launch batched pruning kernel"))
             return 1;
     for (int q = 0; q < nItems; q++) {

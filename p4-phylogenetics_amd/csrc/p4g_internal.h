@@ -136,6 +136,16 @@ struct KidRec {
     int pad;
 };
 constexpr int OP_ROOT = 1;
+// Group of ops processed together: kind 0 = up to G independent binary,
+// non-root ops; kind 1 = one op with any number of children (or the root).
+struct GroupRec {
+    int kind;
+    int nOps;
+    int opStart;
+    int pad;
+};
+// P rows of a program's child branches are staged in LDS when they fit.
+constexpr int LDS_P_BUDGET = 48 * 1024;
 
 // Per-(tree, part) device state.
 struct TreePart {
@@ -162,17 +172,29 @@ struct TreePart {
     double *patLikes = nullptr;    // [nPatPad]
     int maxBlocks = 0;
     // Programs.
-    int *prog = nullptr;           // device ops+kids
+    int *prog = nullptr;           // device groups+ops+kids
     int progCap = 0;
+    int nGroups = 0, nKids = 0;    // of the staged program
     double *rootPi = nullptr;      // [dim]
     // Host-side flags.
     std::vector<char> dirty;       // per node
     std::vector<char> pValid;      // per node
     bool everComputed = false;
-    // Staging: pinned host and a device mirror of the same size.
-    double *hStage = nullptr;
-    double *dStage = nullptr;
-    size_t hStageCap = 0;
+    // Deferred parameter work.  setPrams / calculateBigPDecks snapshot their
+    // inputs here (pinned host memory) and mark them pending; the pending
+    // eigensystems and P matrices are built by one launch each right before
+    // the next pruning launch, on the same stream.
+    double *hEigIn = nullptr;      // [nEig][dim + dim*dim]  pi, R
+    int *hEigList = nullptr;       // [nEig] pending slots (packed at flush)
+    double *dEigIn = nullptr;      // device copy of hEigIn + slot list
+    std::vector<char> eigPending;  // per eigen slot
+    double *hT = nullptr;          // [nNodes][K] effective branch lengths
+    int *hEs = nullptr;            // [nNodes] eigen slot of each node
+    char *hPack = nullptr;         // packed pending P items (t, node, slot)
+    char *dPack = nullptr;
+    size_t packBytes = 0;
+    std::vector<char> pPending;    // per node
+    double *hRootPi = nullptr;     // [MAX_DIM]
     int *hProg = nullptr;
     int hProgCap = 0;
     double *hResult = nullptr;     // pinned [2]
@@ -216,9 +238,10 @@ struct PmatArgs {
 };
 
 struct PruneArgs {
+    const GroupRec *groups;
     const OpRec *ops;
     const KidRec *kids;
-    int nOps;
+    int nGroups, nOps, nKids;
     int dim, K, log2Kp;
     int nPat, nPatPad;
     const uint8_t *tips;
@@ -244,8 +267,10 @@ hipError_t launchEig(const EigArgs &a, hipStream_t s);
 hipError_t launchPmat(const PmatArgs &a, hipStream_t s);
 hipError_t launchPrune(const PruneArgs &a, int precision, int nBlocks, hipStream_t s);
 hipError_t launchPruneBatch(const PruneArgs *dArgs, const int *dBlockItem, int nBlocks,
-                            int dim, int precision, hipStream_t s);
+                            int dim, int precision, size_t lds, hipStream_t s);
 int pruneBlockPatterns(int K);  // patterns per 256-thread block
+int pruneGroupWidth(int dim);   // ops per binary group
+size_t pruneLdsBytes(int nKids, int K, int dim, int precision);  // 0 if P stays in global
 
 // Host numerics (p4g_host.hip).
 int discreteGamma(double *freqK, double *rK, double alfa, double beta, int K, int median);

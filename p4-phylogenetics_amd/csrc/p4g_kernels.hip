@@ -230,6 +230,10 @@ int pruneBlockPatterns(int K)
     return PRUNE_THREADS / Kp;
 }
 
+// Independent binary ops processed together (their loads are issued before
+// any of their arithmetic): more loads in flight per lane.
+int pruneGroupWidth(int dim) { return dim <= 4 ? 4 : 1; }
+
 template <typename T>
 __device__ __forceinline__ T groupMax(T v, int Kp)
 {
@@ -237,175 +241,134 @@ __device__ __forceinline__ T groupMax(T v, int Kp)
     return v;
 }
 
-template <int SC, typename T, bool SCALE>
-__device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk)
+// Contribution of a leaf child (Pf/p4_node.c:660-725, 753-809): column
+// `code` of P, all ones for gap / '?' / an all-state equate, or the sum of
+// the allowed columns for a partial ambiguity.  Pk is P^T: Pk[j*S + s].
+template <int SC, typename T>
+__device__ __forceinline__ void leafVec(const T *Pk, unsigned code, int S, const uint64_t *amb, T *v)
 {
     constexpr int SM = SC ? SC : MAX_DIM;
-    const int S = SC ? SC : a.dim;
-    const int K = a.K;
-    const int lk = a.log2Kp;
-    const int Kp = 1 << lk;
-    const int tid = threadIdx.x;
-    const int kk0 = tid & (Kp - 1);
-    const bool catOn = kk0 < K;
-    const int kk = catOn ? kk0 : K - 1;
-    const int p = blk * (PRUNE_THREADS >> lk) + (tid >> lk);
-    const size_t nPatPad = (size_t)a.nPatPad;
-    const T *PT = (const T *)a.PT;
-    T *cl = (T *)a.cl;
+    if (code < (unsigned)S) {
+#pragma unroll
+        for (int s = 0; s < SM; s++)
+            if (s < S) v[s] = Pk[code * S + s];
+    } else if (code == TIP_ALL) {
+#pragma unroll
+        for (int s = 0; s < SM; s++) v[s] = T(1);
+    } else {
+        const uint64_t m = amb[code - S];
+#pragma unroll
+        for (int s = 0; s < SM; s++) v[s] = T(0);
+        for (int j = 0; j < S; j++)
+            if ((m >> j) & 1ull) {
+#pragma unroll
+                for (int s = 0; s < SM; s++)
+                    if (s < S) v[s] += Pk[j * S + s];
+            }
+    }
+}
 
-    double contrib = 0.0;
-    int bad = 0;
-    for (int o = 0; o < a.nOps; ++o) {
-        const OpRec op = a.ops[o];
-        T acc[SM];
-        int e = 0;
-        for (int c = 0; c < op.nKids; ++c) {
-            const KidRec kd = a.kids[op.kidStart + c];
-            const T *Pt = PT + ((size_t)kd.pnode * K + kk) * S * S;
-            T v[SM];
-            bool skip = false;
-            if (kd.kind == 0) {
-                const unsigned code = a.tips[(size_t)kd.idx * nPatPad + p];
-                if (code < (unsigned)S) {
+// Contribution of an internal child (Pf/p4_node.c:726-744, 810-834):
+// v[s] = sum_j P[s][j] x[j], summed over j in increasing order.
+template <int SC, typename T>
+__device__ __forceinline__ void internalVec(const T *Pk, const T *x, int S, T *v)
+{
+    constexpr int SM = SC ? SC : MAX_DIM;
 #pragma unroll
-                    for (int s = 0; s < SM; s++)
-                        if (s < S) v[s] = Pt[code * S + s];
-                } else if (code == TIP_ALL) {
+    for (int s = 0; s < SM; s++) v[s] = T(0);
 #pragma unroll
-                    for (int s = 0; s < SM; s++) v[s] = T(1);
-                    skip = (c != 0);
-                } else {
-                    const uint64_t m = a.ambMask[code - S];
-#pragma unroll
-                    for (int s = 0; s < SM; s++) v[s] = T(0);
-                    for (int j = 0; j < S; j++)
-                        if ((m >> j) & 1ull) {
-#pragma unroll
-                            for (int s = 0; s < SM; s++)
-                                if (s < S) v[s] += Pt[j * S + s];
-                        }
-                }
-            } else {
-                const T *x = cl + (((size_t)kd.idx * nPatPad + p) * K + kk) * S;
-                T xv[SM];
-#pragma unroll
-                for (int j = 0; j < SM; j++)
-                    if (j < S) xv[j] = x[j];
-#pragma unroll
-                for (int s = 0; s < SM; s++) v[s] = T(0);
-#pragma unroll
-                for (int j = 0; j < SM; j++) {
-                    if (j < S) {
-#pragma unroll
-                        for (int s = 0; s < SM; s++)
-                            if (s < S) v[s] += xv[j] * Pt[j * S + s];
-                    }
-                }
-                if (SCALE) e += a.scl[(size_t)kd.idx * nPatPad + p];
-            }
-            if (c == 0) {
-#pragma unroll
-                for (int s = 0; s < SM; s++) acc[s] = v[s];
-            } else if (!skip) {
-#pragma unroll
-                for (int s = 0; s < SM; s++) acc[s] *= v[s];
-            }
-        }
-        if (SCALE) {
-            T m = T(0);
+    for (int j = 0; j < SM; j++) {
+        if (j < S) {
 #pragma unroll
             for (int s = 0; s < SM; s++)
-                if (s < S) m = fmax(m, acc[s]);
-            m = groupMax(m, Kp);
-            if (m < T(SCALE_THRESH) && m > T(0)) {
-                int ex;
-                frexp(m, &ex);
-                const int E = -ex;
-#pragma unroll
-                for (int s = 0; s < SM; s++)
-                    if (s < S) acc[s] = ldexp(acc[s], E);
-                e += E;
-            }
-        }
-        if (!(op.flags & OP_ROOT)) {
-            if (catOn) {
-                T *dst = cl + (((size_t)op.dst * nPatPad + p) * K + kk) * S;
-#pragma unroll
-                for (int s = 0; s < SM; s++)
-                    if (s < S) dst[s] = acc[s];
-            }
-            if (SCALE && kk0 == 0) a.scl[(size_t)op.dst * nPatPad + p] = e;
-        } else {
-            const double *pi = a.rootPi;
-            double Lk = 0.0;
-            if (a.rootLeafRow >= 0) {
-                const unsigned code = a.tips[(size_t)a.rootLeafRow * nPatPad + p];
-                uint64_t m;
-                if (code < (unsigned)S)
-                    m = 1ull << code;
-                else if (code == TIP_ALL)
-                    m = ~0ull;
-                else
-                    m = a.ambMask[code - S];
-#pragma unroll
-                for (int s = 0; s < SM; s++)
-                    if (s < S && ((m >> s) & 1ull)) Lk += pi[s] * (double)acc[s];
-            } else {
-#pragma unroll
-                for (int s = 0; s < SM; s++)
-                    if (s < S) Lk += pi[s] * (double)acc[s];
-            }
-            if (!catOn) Lk = 0.0;
-            for (int off = 1; off < Kp; off <<= 1) Lk += __shfl_xor(Lk, off);
-            if (kk0 == 0) {
-                // Site likelihood as Pf/p4_tree.c:1029-1197.  In FP32 mode
-                // the partials carry 2^-e, so the variable-site part is
-                // unscaled in FP64 before the invariant part is added; with
-                // no invariant part the log is taken in the scaled domain.
-                const double LN2 = 0.69314718055994530942;
-                double like, lnl;
-                bool ok;
-                if (a.pInvar != 0.0) {
-                    const double varS = Lk * ((1.0 - a.pInvar) / (double)K);
-                    double inv = 0.0;
-                    for (int w = 0; w < a.nWords; w++) {
-                        uint32_t mw = a.invMask[(size_t)w * nPatPad + p];
-                        while (mw) {
-                            const int b = __ffs(mw) - 1;
-                            mw &= mw - 1;
-                            inv += pi[w * 32 + b] * a.pInvar;
-                        }
-                    }
-                    if (SCALE && e != 0 && inv == 0.0) {
-                        ok = varS > 0.0;
-                        lnl = log(varS) - (double)e * LN2;
-                        like = ldexp(varS, -e);
-                    } else {
-                        like = ((SCALE && e != 0) ? ldexp(varS, -e) : varS) + inv;
-                        ok = like > 0.0;
-                        lnl = log(like);
-                    }
-                } else {
-                    const double likeS = (K > 1) ? Lk / (double)K : Lk;
-                    ok = likeS > 0.0;
-                    lnl = log(likeS);
-                    like = likeS;
-                    if (SCALE && e != 0) {
-                        lnl -= (double)e * LN2;
-                        like = ldexp(likeS, -e);
-                    }
-                }
-                if (p < a.nPat) {
-                    if (!ok) bad = 1;
-                    contrib = a.counts[p] * lnl;
-                    if (a.patLikes) a.patLikes[p] = like;
-                }
-            }
+                if (s < S) v[s] += x[j] * Pk[j * S + s];
         }
     }
+}
 
-    // workgroup reduction, then the last workgroup of this item sums all
+// FP32 rescaling: when the largest entry of the (pattern, all categories)
+// vector is below 2^-24, multiply by the power of two that brings it to
+// [0.5, 1) and add that power to the node's exponent.  Exact.
+template <int SC, typename T>
+__device__ __forceinline__ void rescale(T *acc, int &e, int S, int Kp)
+{
+    constexpr int SM = SC ? SC : MAX_DIM;
+    T m = T(0);
+#pragma unroll
+    for (int s = 0; s < SM; s++)
+        if (s < S) m = fmax(m, acc[s]);
+    m = groupMax(m, Kp);
+    if (m < T(SCALE_THRESH) && m > T(0)) {
+        int ex;
+        frexp(m, &ex);
+        const int E = -ex;
+#pragma unroll
+        for (int s = 0; s < SM; s++)
+            if (s < S) acc[s] = ldexp(acc[s], E);
+        e += E;
+    }
+}
+
+// Site likelihood of pattern p from the root's category-summed
+// sum_s pi_s cl_s (Lk) and the pattern's scale exponent e, as
+// Pf/p4_tree.c:1029-1197: (1 - pInvar)/K * Lk plus the invariant-site term,
+// or Lk/K without pInvar; adds count * log(like) to contrib.  In FP32 mode
+// the partials carry 2^-e, so the variable-site part is unscaled in FP64
+// before the invariant part is added; with no invariant part the log is
+// taken in the scaled domain.
+template <bool SCALE>
+__device__ __forceinline__ void siteLogLike(const PruneArgs &a, double Lk, int e, int p, int K,
+                                            const double *pi, double &contrib, int &bad)
+{
+    const size_t nPatPad = (size_t)a.nPatPad;
+    const double LN2 = 0.69314718055994530942;
+    double like, lnl;
+    bool ok;
+    if (a.pInvar != 0.0) {
+        const double varS = Lk * ((1.0 - a.pInvar) / (double)K);
+        double inv = 0.0;
+        for (int w = 0; w < a.nWords; w++) {
+            uint32_t mw = a.invMask[(size_t)w * nPatPad + p];
+            while (mw) {
+                const int b = __ffs(mw) - 1;
+                mw &= mw - 1;
+                inv += pi[w * 32 + b] * a.pInvar;
+            }
+        }
+        if (SCALE && e != 0 && inv == 0.0) {
+            ok = varS > 0.0;
+            lnl = log(varS) - (double)e * LN2;
+            like = ldexp(varS, -e);
+        } else {
+            like = ((SCALE && e != 0) ? ldexp(varS, -e) : varS) + inv;
+            ok = like > 0.0;
+            lnl = log(like);
+        }
+    } else {
+        const double likeS = (K > 1) ? Lk / (double)K : Lk;
+        ok = likeS > 0.0;
+        lnl = log(likeS);
+        like = likeS;
+        if (SCALE && e != 0) {
+            lnl -= (double)e * LN2;
+            like = ldexp(likeS, -e);
+        }
+    }
+    if (p < a.nPat) {
+        if (!ok) bad = 1;
+        contrib = a.counts[p] * lnl;
+        if (a.patLikes) a.patLikes[p] = like;
+    }
+}
+
+// Workgroup reduction of the per-lane weighted site log likelihoods, then
+// the last workgroup of the item (atomic ticket) sums the per-workgroup
+// partials in a fixed order with Neumaier compensation and resets the
+// ticket for the next launch.
+__device__ __forceinline__ void finishSiteSum(const PruneArgs &a, double contrib, int bad, int blk,
+                                              int nBlk)
+{
+    const int tid = threadIdx.x;
     for (int off = 32; off >= 1; off >>= 1) {
         contrib += __shfl_xor(contrib, off);
         bad |= __shfl_xor(bad, off);
@@ -454,6 +417,7 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk)
             v += __shfl_xor(v, off);
             b |= __shfl_xor(b, off);
         }
+        __syncthreads();
         if (lane == 0) {
             wsum[wave] = v;
             wbad[wave] = b;
@@ -473,71 +437,253 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk)
     }
 }
 
-template <int SC, typename T, bool SCALE>
+
+template <int SC, typename T, bool SCALE, bool PLDS, int G>
+__device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk, T *sP)
+{
+    constexpr int SM = SC ? SC : MAX_DIM;
+    const int S = SC ? SC : a.dim;
+    const int K = a.K;
+    const int lk = a.log2Kp;
+    const int Kp = 1 << lk;
+    const int tid = threadIdx.x;
+    const int kk0 = tid & (Kp - 1);
+    const bool catOn = kk0 < K;
+    const int kk = catOn ? kk0 : K - 1;
+    const int p = blk * (PRUNE_THREADS >> lk) + (tid >> lk);
+    const size_t nPatPad = (size_t)a.nPatPad;
+    const size_t SS = (size_t)S * S;
+    const T *PTg = (const T *)a.PT;
+    T *cl = (T *)a.cl;
+
+    if (PLDS) {
+        // P^T of every child branch the program uses, [kid][K][S][S]
+        const int per = K * S * S;
+        const int total = a.nKids * per;
+        for (int i = tid; i < total; i += PRUNE_THREADS) {
+            const int kid = i / per;
+            sP[i] = PTg[(size_t)a.kids[kid].pnode * per + (i - kid * per)];
+        }
+        __syncthreads();
+    }
+    auto Pof = [&](int kidIdx, int pnode) -> const T * {
+        return PLDS ? sP + ((size_t)kidIdx * K + kk) * SS : PTg + ((size_t)pnode * K + kk) * SS;
+    };
+
+    double contrib = 0.0;
+    int bad = 0;
+    for (int g = 0; g < a.nGroups; ++g) {
+        const GroupRec gr = a.groups[g];
+        if (gr.kind == 0) {
+            // ---- up to G independent binary, non-root ops
+            T xv[G][2][SM];
+            unsigned code[G][2];
+            int ek[G];
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                ek[j] = 0;
+                if (j < gr.nOps) {
+                    const OpRec op = a.ops[gr.opStart + j];
+#pragma unroll
+                    for (int c = 0; c < 2; c++) {
+                        const KidRec kd = a.kids[op.kidStart + c];
+                        if (kd.kind == 0) {
+                            code[j][c] = a.tips[(size_t)kd.idx * nPatPad + p];
+                        } else {
+                            code[j][c] = 0u;
+                            const T *x = cl + (((size_t)kd.idx * nPatPad + p) * K + kk) * S;
+#pragma unroll
+                            for (int s = 0; s < SM; s++)
+                                if (s < S) xv[j][c][s] = x[s];
+                            if (SCALE) ek[j] += a.scl[(size_t)kd.idx * nPatPad + p];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                if (j < gr.nOps) {
+                    const OpRec op = a.ops[gr.opStart + j];
+                    T acc[SM], v[SM];
+#pragma unroll
+                    for (int c = 0; c < 2; c++) {
+                        const KidRec kd = a.kids[op.kidStart + c];
+                        const T *Pk = Pof(op.kidStart + c, kd.pnode);
+                        if (kd.kind == 0)
+                            leafVec<SC, T>(Pk, code[j][c], S, a.ambMask, v);
+                        else
+                            internalVec<SC, T>(Pk, xv[j][c], S, v);
+                        if (c == 0) {
+#pragma unroll
+                            for (int s = 0; s < SM; s++) acc[s] = v[s];
+                        } else {
+#pragma unroll
+                            for (int s = 0; s < SM; s++) acc[s] *= v[s];
+                        }
+                    }
+                    int e = ek[j];
+                    if (SCALE) rescale<SC, T>(acc, e, S, Kp);
+                    if (catOn) {
+                        T *dst = cl + (((size_t)op.dst * nPatPad + p) * K + kk) * S;
+#pragma unroll
+                        for (int s = 0; s < SM; s++)
+                            if (s < S) dst[s] = acc[s];
+                    }
+                    if (SCALE && kk0 == 0) a.scl[(size_t)op.dst * nPatPad + p] = e;
+                }
+            }
+            continue;
+        }
+        // ---- one op with any number of children, possibly the root
+        const OpRec op = a.ops[gr.opStart];
+        T acc[SM];
+        int e = 0;
+        for (int c = 0; c < op.nKids; ++c) {
+            const KidRec kd = a.kids[op.kidStart + c];
+            const T *Pk = Pof(op.kidStart + c, kd.pnode);
+            T v[SM];
+            bool skip = false;
+            if (kd.kind == 0) {
+                const unsigned code = a.tips[(size_t)kd.idx * nPatPad + p];
+                leafVec<SC, T>(Pk, code, S, a.ambMask, v);
+                skip = (code == TIP_ALL) && c != 0;
+            } else {
+                internalVec<SC, T>(Pk, cl + (((size_t)kd.idx * nPatPad + p) * K + kk) * S, S, v);
+                if (SCALE) e += a.scl[(size_t)kd.idx * nPatPad + p];
+            }
+            if (c == 0) {
+#pragma unroll
+                for (int s = 0; s < SM; s++) acc[s] = v[s];
+            } else if (!skip) {
+#pragma unroll
+                for (int s = 0; s < SM; s++) acc[s] *= v[s];
+            }
+        }
+        if (SCALE) rescale<SC, T>(acc, e, S, Kp);
+        if (!(op.flags & OP_ROOT)) {
+            if (catOn) {
+                T *dst = cl + (((size_t)op.dst * nPatPad + p) * K + kk) * S;
+#pragma unroll
+                for (int s = 0; s < SM; s++)
+                    if (s < S) dst[s] = acc[s];
+            }
+            if (SCALE && kk0 == 0) a.scl[(size_t)op.dst * nPatPad + p] = e;
+            continue;
+        }
+        // ---- root: site likelihood and weighted log (Pf/p4_tree.c:1029-1378)
+        const double *pi = a.rootPi;
+        double Lk = 0.0;
+        if (a.rootLeafRow >= 0) {
+            const unsigned code = a.tips[(size_t)a.rootLeafRow * nPatPad + p];
+            uint64_t m;
+            if (code < (unsigned)S)
+                m = 1ull << code;
+            else if (code == TIP_ALL)
+                m = ~0ull;
+            else
+                m = a.ambMask[code - S];
+#pragma unroll
+            for (int s = 0; s < SM; s++)
+                if (s < S && ((m >> s) & 1ull)) Lk += pi[s] * (double)acc[s];
+        } else {
+#pragma unroll
+            for (int s = 0; s < SM; s++)
+                if (s < S) Lk += pi[s] * (double)acc[s];
+        }
+        if (!catOn) Lk = 0.0;
+        for (int off = 1; off < Kp; off <<= 1) Lk += __shfl_xor(Lk, off);
+        if (kk0 == 0) siteLogLike<SCALE>(a, Lk, e, p, K, pi, contrib, bad);
+    }
+    finishSiteSum(a, contrib, bad, blk, nBlk);
+}
+
+template <int SC, typename T, bool SCALE, bool PLDS>
 __global__ __launch_bounds__(PRUNE_THREADS) void k_prune(PruneArgs a)
 {
-    pruneBody<SC, T, SCALE>(a, blockIdx.x, gridDim.x);
+    extern __shared__ unsigned char smemRaw[];
+    constexpr int G = (SC == 2 || SC == 4) ? 4 : 1;
+    pruneBody<SC, T, SCALE, PLDS, G>(a, blockIdx.x, gridDim.x, reinterpret_cast<T *>(smemRaw));
 }
 
 // Batched variant: several (tree, part) items of the same state count and
-// precision in one launch.  blockItem[b] = item index; the item's own first
-// block index is found from its tile offset.
-struct BatchItemHdr {
-    int firstBlock, nBlocks;
-};
-
-template <int SC, typename T, bool SCALE>
+// precision in one launch; blockItem[2b] = item, blockItem[2b+1] = the
+// block's index within the item.
+template <int SC, typename T, bool SCALE, bool PLDS>
 __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneBatch(const PruneArgs *args,
                                                               const int *blockItem)
 {
+    extern __shared__ unsigned char smemRaw[];
+    constexpr int G = (SC == 2 || SC == 4) ? 4 : 1;
     const int item = blockItem[2 * blockIdx.x];
     const int local = blockItem[2 * blockIdx.x + 1];
     const PruneArgs a = args[item];
     const int nBlk = a.nPatPad / (PRUNE_THREADS >> a.log2Kp);
-    pruneBody<SC, T, SCALE>(a, local, nBlk);
+    pruneBody<SC, T, SCALE, PLDS, G>(a, local, nBlk, reinterpret_cast<T *>(smemRaw));
+}
+
+template <int SC, typename T, bool SCALE>
+static hipError_t launchPruneP(const PruneArgs &a, size_t lds, int nBlocks, hipStream_t s)
+{
+    // lds <= LDS_P_BUDGET (48 KiB), within the default per-workgroup limit
+    if (lds) {
+        hipLaunchKernelGGL((k_prune<SC, T, SCALE, true>), dim3(nBlocks), dim3(PRUNE_THREADS), lds, s, a);
+    } else {
+        hipLaunchKernelGGL((k_prune<SC, T, SCALE, false>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a);
+    }
+    return hipGetLastError();
 }
 
 template <int SC>
-static hipError_t launchPruneS(const PruneArgs &a, int precision, int nBlocks, hipStream_t s)
+static hipError_t launchPruneS(const PruneArgs &a, int precision, size_t lds, int nBlocks, hipStream_t s)
 {
-    if (precision == FP32)
-        hipLaunchKernelGGL((k_prune<SC, float, true>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_prune<SC, double, false>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a);
-    return hipGetLastError();
+    if (precision == FP32) return launchPruneP<SC, float, true>(a, lds, nBlocks, s);
+    return launchPruneP<SC, double, false>(a, lds, nBlocks, s);
+}
+
+size_t pruneLdsBytes(int nKids, int K, int dim, int precision)
+{
+    const size_t b = (size_t)nKids * K * dim * dim * (precision == FP32 ? 4 : 8);
+    return b <= (size_t)LDS_P_BUDGET ? b : 0;
 }
 
 hipError_t launchPrune(const PruneArgs &a, int precision, int nBlocks, hipStream_t s)
 {
+    const size_t lds = pruneLdsBytes(a.nKids, a.K, a.dim, precision);
     switch (a.dim) {
-    case 2: return launchPruneS<2>(a, precision, nBlocks, s);
-    case 4: return launchPruneS<4>(a, precision, nBlocks, s);
-    case 20: return launchPruneS<20>(a, precision, nBlocks, s);
-    default: return launchPruneS<0>(a, precision, nBlocks, s);
+    case 2: return launchPruneS<2>(a, precision, lds, nBlocks, s);
+    case 4: return launchPruneS<4>(a, precision, lds, nBlocks, s);
+    case 20: return launchPruneS<20>(a, precision, lds, nBlocks, s);
+    default: return launchPruneS<0>(a, precision, lds, nBlocks, s);
     }
 }
 
-template <int SC>
-static hipError_t launchBatchS(const PruneArgs *d, const int *bi, int nBlocks, int precision,
-                               hipStream_t s)
+template <int SC, typename T, bool SCALE>
+static hipError_t launchBatchP(const PruneArgs *d, const int *bi, int nBlocks, size_t lds, hipStream_t s)
 {
-    if (precision == FP32)
-        hipLaunchKernelGGL((k_pruneBatch<SC, float, true>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s,
-                           d, bi);
-    else
-        hipLaunchKernelGGL((k_pruneBatch<SC, double, false>), dim3(nBlocks), dim3(PRUNE_THREADS), 0,
-                           s, d, bi);
+    if (lds) {
+        hipLaunchKernelGGL((k_pruneBatch<SC, T, SCALE, true>), dim3(nBlocks), dim3(PRUNE_THREADS), lds, s, d, bi);
+    } else {
+        hipLaunchKernelGGL((k_pruneBatch<SC, T, SCALE, false>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi);
+    }
     return hipGetLastError();
 }
 
+template <int SC>
+static hipError_t launchBatchS(const PruneArgs *d, const int *bi, int nBlocks, int precision, size_t lds,
+                               hipStream_t s)
+{
+    if (precision == FP32) return launchBatchP<SC, float, true>(d, bi, nBlocks, lds, s);
+    return launchBatchP<SC, double, false>(d, bi, nBlocks, lds, s);
+}
+
 hipError_t launchPruneBatch(const PruneArgs *dArgs, const int *dBlockItem, int nBlocks, int dim,
-                            int precision, hipStream_t s)
+                            int precision, size_t lds, hipStream_t s)
 {
     switch (dim) {
-    case 2: return launchBatchS<2>(dArgs, dBlockItem, nBlocks, precision, s);
-    case 4: return launchBatchS<4>(dArgs, dBlockItem, nBlocks, precision, s);
-    case 20: return launchBatchS<20>(dArgs, dBlockItem, nBlocks, precision, s);
-    default: return launchBatchS<0>(dArgs, dBlockItem, nBlocks, precision, s);
+    case 2: return launchBatchS<2>(dArgs, dBlockItem, nBlocks, precision, lds, s);
+    case 4: return launchBatchS<4>(dArgs, dBlockItem, nBlocks, precision, lds, s);
+    case 20: return launchBatchS<20>(dArgs, dBlockItem, nBlocks, precision, lds, s);
+    default: return launchBatchS<0>(dArgs, dBlockItem, nBlocks, precision, lds, s);
     }
 }
 

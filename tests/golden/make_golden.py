@@ -1,0 +1,168 @@
+"""Write the golden vectors in tests/golden/ from the compiled reference
+subset (oracle/_ref/libp4ref.so, built by oracle/Makefile from the
+reference's Pf/*.c sources).  Runs only where the reference sources are
+present; the vectors it writes are committed.
+
+    make -C oracle && python tests/golden/make_golden.py
+
+Each likelihood fixture holds the full input (topology, node model numbers,
+model values, coded patterns and counts) and the reference's outputs
+(lnL, per-pattern likelihoods, gamma rates, P matrices of a few nodes).
+"""
+import os
+import sys
+
+import numpy
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "p4-phylogenetics_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import oracle  # noqa: E402
+from p4gpu import func, synth  # noqa: E402
+from p4gpu.data import DNA_EQUATES, DNA_SYMBOLS  # noqa: E402
+
+
+def save_case(name, f, dataType, extra=None):
+    v, pl, Pm, cl, rates = oracle.ref_part_loglike(f)
+    arrs = dict(parent=f.parent, leftChild=f.leftChild, sibling=f.sibling, isLeaf=f.isLeaf,
+                seqNum=f.seqNum, brLen=f.brLen, postOrder=f.postOrder, root=numpy.int32(f.root),
+                compNums=f.compNums, rMatrixNums=f.rMatrixNums, gdasrvNums=f.gdasrvNums,
+                dim=numpy.int32(f.dim), nCat=numpy.int32(f.nCat), comps=f.comps, bigR=f.bigR,
+                alphas=f.alphas, pInvar=numpy.float64(f.pInvar), relRate=numpy.float64(f.relRate),
+                patterns=f.patterns, counts=f.counts, equates=f.equates,
+                dataType=numpy.array(dataType), lnL=numpy.float64(v), patternLikes=pl,
+                rates=rates, P=Pm[:min(6, len(Pm))])
+    if extra:
+        arrs.update(extra)
+    numpy.savez_compressed(os.path.join(HERE, name + ".npz"), **arrs)
+    print("%-22s nPat %5d  lnL %.12f" % (name, f.patterns.shape[1], v))
+
+
+def workload_case(name, wl, seed, nSites):
+    tree, data = synth.make_workload(wl, seed=seed, nSites=nSites)
+    f = oracle.flat_from_tree(tree, 0)
+    save_case(name, f, data.parts[0].dataType)
+
+
+def edge_dna_case():
+    """8 taxa with gaps, '?', 'n', partial ambiguities (r, y, h, ...), an
+    all-gap column and constant columns; GTR+G4 with pInvar."""
+    rng = numpy.random.default_rng(11)
+    t = func.randomTree(nTax=8, seed=11, meanBrLen=0.15)
+    names = t.taxNames
+    alpha = list(DNA_SYMBOLS) + ['-', '?', 'n', 'r', 'y', 'h', 'k', 'w']
+    seqs = []
+    for i in range(8):
+        s = ''.join(rng.choice(list(DNA_SYMBOLS), 300))
+        s = list(s)
+        for j in rng.choice(300, 40, replace=False):
+            s[j] = rng.choice(alpha[4:])
+        s[5] = '-'          # column 5 all gaps
+        s[6] = 'a'          # column 6 constant a
+        s[7] = 'r' if i % 2 else 'a'   # column 7 invariant under r
+        seqs.append(''.join(s))
+    codes = oracle.code_sequences(seqs, DNA_SYMBOLS, DNA_EQUATES)
+    pats, cnts, _ = oracle.compress(codes)
+    from p4gpu.data import Alignment, Data
+    t.data = Data(Alignment(names, seqs, 'dna'))
+    t.newComp(free=1, spec='specified', val=[0.3, 0.2, 0.15, 0.35])
+    t.newRMatrix(free=1, spec='specified', val=[1.0, 2.5, 0.7, 0.9, 3.1, 1.2])
+    t.setNGammaCat(nGammaCat=4)
+    t.newGdasrv(free=1, val=0.7)
+    t.setPInvar(free=1, val=0.25)
+    t.modelSanityCheck()
+    f = oracle.flat_from_tree(t, 0, patterns=pats, counts=cnts)
+    save_case("edge_dna_gaps_pinv", f, "dna", extra=dict(sequences=numpy.array(seqs)))
+
+
+def multifurcation_case():
+    t = func.readNewick("((a:0.1,b:0.2,c:0.05):0.1,(d:0.3,e:0.1):0.2,f:0.4,g:0.01);")
+    _simple_dna(t, "multifurcation_root4", seed=12, K=1, pinv=0.0)
+
+
+def leafroot_case():
+    # rooted on leaf 'a': the root is a leaf with one child (Pf/p4_tree.c:1199-1378)
+    t = func.readNewick("((b:0.1,c:0.2):0.05,(d:0.3,e:0.1):0.2,a:0.15);")
+    old = t.root
+    a = t.node("a")
+    # detach a from the old root's child list; a becomes the root, the old
+    # root its only child, on a's branch
+    kids = [c for c in old.iterChildren() if c is not a]
+    old.leftChild = kids[0]
+    for x, y in zip(kids, kids[1:] + [None]):
+        x.sibling = y
+    old.parent = a
+    old.sibling = None
+    old.br.len = a.br.len
+    a.parent = None
+    a.sibling = None
+    a.leftChild = old
+    t.root = a
+    t.setPreAndPostOrder()
+    _simple_dna(t, "leaf_root", seed=13, K=4, pinv=0.1)
+
+
+def _simple_dna(t, name, seed, K, pinv):
+    from p4gpu.data import Alignment, Data
+    rng = numpy.random.default_rng(seed)
+    names = sorted(n.name for n in t.nodes if n.name)
+    seqs = [''.join(rng.choice(list('acgt') + ['-', 'r'], 200, p=[.23] * 4 + [.04, .04]))
+            for _ in names]
+    t.data = Data(Alignment(names, seqs, 'dna'))
+    t.newComp(free=1, spec='specified', val=[0.1, 0.4, 0.3, 0.2])
+    t.newRMatrix(free=1, spec='specified', val=[2.0, 1.0, 0.5, 1.5, 4.0, 1.0])
+    t.setNGammaCat(nGammaCat=K)
+    if K > 1:
+        t.newGdasrv(free=1, val=1.3)
+    t.setPInvar(free=1, val=pinv)
+    t.modelSanityCheck()
+    codes = oracle.code_sequences(seqs, DNA_SYMBOLS, DNA_EQUATES)
+    pats, cnts, _ = oracle.compress(codes)
+    f = oracle.flat_from_tree(t, 0, patterns=pats, counts=cnts)
+    save_case(name, f, "dna", extra=dict(sequences=numpy.array(seqs), names=numpy.array(names)))
+
+
+def gamma_vectors():
+    alphas = numpy.array([0.1, 0.2, 0.3, 0.5, 0.75, 1.0, 1.5, 2.0, 5.0, 20.0, 100.0, 299.0])
+    Ks = numpy.array([2, 3, 4, 5, 8, 16])
+    rates = numpy.zeros((len(alphas), len(Ks), 16))
+    for i, a in enumerate(alphas):
+        for j, K in enumerate(Ks):
+            rates[i, j, :K] = oracle.discrete_gamma(float(a), int(K), use_ref=True)
+    numpy.savez_compressed(os.path.join(HERE, "gamma_rates.npz"), alphas=alphas, Ks=Ks,
+                           rates=rates)
+    print("gamma_rates            %d x %d" % (len(alphas), len(Ks)))
+
+
+def pmat_vectors():
+    rng = numpy.random.default_rng(21)
+    out = {}
+    ts = numpy.array([0.0, 1e-8, 1e-5, 0.01, 0.1, 0.5, 1.0, 3.0, 10.0])
+    for S, tag in ((4, "dna"), (20, "aa")):
+        for rep in range(2):
+            pi = rng.dirichlet([1.0] * S)
+            pi = numpy.maximum(pi, 1e-6)
+            pi /= pi.sum()
+            pi[-1] = 1.0 - pi[:-1].sum()
+            R = synth.bigR_from_vals(rng.lognormal(0, 1.0, S * (S - 1) // 2), S)
+            Q, P = oracle.ref_pmat(R, pi, ts)
+            k = "%s%d" % (tag, rep)
+            out[k + "_pi"], out[k + "_R"], out[k + "_Q"], out[k + "_P"] = pi, R, Q, P
+    out["ts"] = ts
+    numpy.savez_compressed(os.path.join(HERE, "pmat.npz"), **out)
+    print("pmat                   4- and 20-state, %d branch lengths" % len(ts))
+
+
+if __name__ == "__main__":
+    if not oracle.ref_available():
+        sys.exit("oracle/_ref/libp4ref.so is missing: run make -C oracle where the reference is present")
+    gamma_vectors()
+    pmat_vectors()
+    workload_case("gtr16_500", "gtr16", seed=101, nSites=500)
+    workload_case("het64_800", "het64", seed=102, nSites=800)
+    workload_case("prot128_150", "prot128", seed=103, nSites=150)
+    edge_dna_case()
+    multifurcation_case()
+    leafroot_case()

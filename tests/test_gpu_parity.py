@@ -1,0 +1,237 @@
+"""GPU parity of libp4gpu against the reference's golden vectors and the
+oracle.  Tolerances: FP64 path 1e-10 relative on lnL (the project target),
+FP32 path with scale exponents 1e-6 relative on lnL."""
+import numpy
+import pytest
+
+import oracle
+from helpers import LNL_CASES, load, oracle_flat, tree_from_fixture
+from p4gpu import Alignment, Data, P4Error, func, pf, synth
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 1e-6
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    assert pf.deviceAvailable(), "libp4gpu sees no GPU"
+
+
+# --------------------------------------------------------- golden vectors
+@pytest.mark.parametrize("case", LNL_CASES)
+def test_golden_fp64(case):
+    fx = load(case)
+    t = tree_from_fixture(fx, "fp64")
+    v = t.calcLogLike(verbose=0)
+    assert rel(v, float(fx["lnL"])) < FP64_TOL, (v, float(fx["lnL"]))
+    sl = numpy.array(t.getSiteLikes())
+    # a part built from patterns lists its sites in pattern order
+    want = numpy.repeat(fx["patternLikes"], fx["counts"])
+    assert numpy.allclose(sl, want, rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("case", LNL_CASES)
+def test_golden_fp32(case):
+    fx = load(case)
+    t = tree_from_fixture(fx, "fp32")
+    v = t.calcLogLike(verbose=0)
+    assert rel(v, float(fx["lnL"])) < FP32_TOL, (v, float(fx["lnL"]))
+
+
+@pytest.mark.parametrize("case", ["gtr16_500", "het64_800", "prot128_150"])
+def test_golden_p_matrices(case):
+    fx = load(case)
+    t = tree_from_fixture(fx, "fp64")
+    t.calcLogLike(verbose=0)
+    K, S = int(fx["nCat"]), int(fx["dim"])
+    for n in range(1, len(fx["P"])):
+        if n == int(fx["root"]):
+            continue
+        got = pf.p4_getBigPDeck(t.nodes[n].cNode, 0, K, S)
+        assert numpy.abs(got - fx["P"][n]).max() < 2e-14, n
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+def test_partials_match_oracle(prec, tol):
+    """Every stored partial, unscaled, against the oracle's.  The error
+    bound is relative to the largest entry of the same pattern (FP32 entries
+    far below that largest one lose relative precision, harmlessly)."""
+    fx = load("het64_800")
+    t = tree_from_fixture(fx, prec)
+    t.calcLogLike(verbose=0)
+    f = oracle_flat(fx)
+    _, _, cl = oracle.oracle_cl(f)
+    K, S, nP = int(fx["nCat"]), int(fx["dim"]), f.patterns.shape[1]
+    for n in t.nodes:
+        if n.isLeaf or n is t.root:
+            continue
+        got = pf.p4_getCondLikes(n.cNode, 0, K, S, nP)
+        want = cl[n.nodeNum]
+        scale = want.max(axis=(0, 1))[None, None, :]
+        assert (numpy.abs(got - want) <= tol * scale).all(), n.nodeNum
+
+
+# -------------------------------------------------- full-size workloads
+@pytest.mark.parametrize("name,nSites", [("gtr16", None), ("het64", None), ("prot128", 4000)])
+def test_workloads_vs_oracle(name, nSites):
+    tree, _ = synth.make_workload(name, seed=11, nSites=nSites)
+    want = oracle.part_loglike(oracle.flat_from_tree(tree, 0))
+    v64 = tree.calcLogLike(verbose=0)
+    tree.setPrecision("fp32")
+    v32 = tree.calcLogLike(verbose=0)
+    tree.setPrecision("fp64")
+    again = tree.calcLogLike(verbose=0)
+    assert rel(v64, want) < FP64_TOL
+    assert rel(v32, want) < FP32_TOL
+    assert again == v64
+
+
+def test_full_size_protein_fp32_vs_fp64():
+    tree, _ = synth.make_workload("prot128", seed=12)
+    v64 = tree.calcLogLike(verbose=0)
+    tree.setPrecision("fp32")
+    v32 = tree.calcLogLike(verbose=0)
+    assert rel(v32, v64) < FP32_TOL
+
+
+def test_multipart_tree():
+    tree, data = synth.make_workload("multi48", seed=3, nSites=4000)
+    v = tree.calcLogLike(verbose=0)
+    parts = [oracle.part_loglike(oracle.flat_from_tree(tree, p)) for p in range(data.nParts)]
+    assert rel(v, sum(parts)) < FP64_TOL
+    assert numpy.allclose(tree.partLikes, parts, rtol=FP64_TOL)
+
+
+# ----------------------------------------------- incremental evaluation
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_dirty_path_equals_full(prec):
+    tree, _ = synth.make_workload("het64", seed=4, nSites=3000, precision=prec)
+    tree.calcLogLike(verbose=0)
+    c = tree.cTree
+    rng = numpy.random.default_rng(0)
+    for _ in range(5):
+        n = tree.nodes[int(rng.integers(1, len(tree.nodes)))]
+        n.br.len *= float(rng.uniform(0.5, 2.0))
+        pf.p4_setBrLen(n.cNode, n.br.len)
+        pf.p4_calculateBigPDecks(n.cNode)
+        p = n.parent
+        while p is not None:
+            pf.p4_setConditionalLikelihoodsOfInternalNodePart(p.cNode, 0)
+            p = p.parent
+        inc = pf.p4_partLogLike(c, tree.data.parts[0].cPart, 0, 0)
+        full = pf.p4_treeLogLike(c, 0)
+        assert rel(inc, full) < (1e-13 if prec == "fp64" else 1e-9), (inc, full)
+
+
+def test_batch_equals_single():
+    trees = []
+    for seed, prec in [(1, "fp64"), (2, "fp32"), (3, "fp32")]:
+        t, _ = synth.make_workload("gtr16", seed=seed, nSites=1500, precision=prec)
+        t.calcLogLike(verbose=0)
+        trees.append(t)
+    t4, _ = synth.make_workload("multi48", seed=5, nSites=2000, precision="fp32")
+    t4.calcLogLike(verbose=0)
+    trees.append(t4)
+    single = [pf.p4_treeLogLike(t.cTree, 0) for t in trees]
+    for t in trees:
+        for n in t.iterInternals():
+            pf.p4_setConditionalLikelihoodsOfInternalNode(n.cNode)
+    b = pf.Batch([t.cTree for t in trees])
+    got = b.logLikes()
+    for g, s in zip(got, single):
+        assert rel(g, s) < 1e-13
+
+
+def test_copy_and_verify():
+    a, _ = synth.make_workload("gtr16", seed=8, nSites=800)
+    a.calcLogLike(verbose=0)
+    b = a.dupe()
+    b.calcLogLike(verbose=0)
+    assert pf.p4_verifyIdentityOfTwoTrees(a.cTree, b.cTree) == 0
+    n = b.nodes[3]
+    n.br.len *= 1.7
+    b.calcLogLike(verbose=0)
+    assert pf.p4_verifyIdentityOfTwoTrees(a.cTree, b.cTree) == 1
+    # copy a's state into b: identical again
+    n.br.len /= 1.7
+    pf.p4_setBrLen(n.cNode, n.br.len)
+    pf.p4_copyCondLikes(a.cTree, b.cTree, 1)
+    pf.p4_copyBigPDecks(a.cTree, b.cTree, 1)
+    pf.p4_copyModelPrams(a.cTree, b.cTree)
+    assert pf.p4_verifyIdentityOfTwoTrees(a.cTree, b.cTree) == 0
+    assert pf.p4_partLogLike(b.cTree, b.data.parts[0].cPart, 0, 0) == pf.p4_treeLogLike(a.cTree, 0)
+
+
+# ------------------------------------------------------------ error paths
+def test_comp_below_pivec_min_raises():
+    tree, _ = synth.make_workload("gtr16", seed=9, nSites=200)
+    tree.calcLogLike(verbose=0)
+    c = tree.model.parts[0].comps[0]
+    c.val[:] = [1e-15, 0.5, 0.25, 0.25 - 1e-15]
+    with pytest.raises(P4Error, match="Bad"):
+        pf.p4_setPrams(tree.cTree, -1)
+
+
+def test_comp_not_summing_to_one_raises():
+    tree, _ = synth.make_workload("gtr16", seed=9, nSites=200)
+    tree.calcLogLike(verbose=0)
+    tree.model.parts[0].comps[0].val[:] = [0.3, 0.3, 0.3, 0.3]
+    with pytest.raises(P4Error, match="sum to 1"):
+        pf.p4_setPrams(tree.cTree, -1)
+
+
+def test_underflow_sentinel_fp64_and_scaling_fp32():
+    """FP64 without scaling underflows on a 300-taxon, long-branch protein
+    tree and returns the reference's -1e99; FP32 with scale exponents
+    returns a finite value."""
+    rng = numpy.random.default_rng(1)
+    t = func.randomTree(nTax=300, seed=1, meanBrLen=2.5)
+    names = t.taxNames
+    seqs = [''.join(rng.choice(list('arndcqeghilkmfpstwyv'), 40)) for _ in names]
+    t.data = Data(Alignment(names, seqs, 'protein'))
+    t.newComp(free=1, spec='equal')
+    t.newRMatrix(free=1, spec='ones')
+    v = t.calcLogLike(verbose=0)
+    assert v == -1.0e99
+    t.setPrecision("fp32")
+    v32 = t.calcLogLike(verbose=0)
+    assert -1e7 < v32 < 0
+
+
+# ------------------------------------------------ other state counts, K
+@pytest.mark.parametrize("symbols", ["01", "012345"])
+def test_standard_data_generic_state_count(symbols):
+    rng = numpy.random.default_rng(len(symbols))
+    t = func.randomTree(nTax=10, seed=len(symbols), meanBrLen=0.2)
+    names = t.taxNames
+    seqs = [''.join(rng.choice(list(symbols) + ['-'], 300)) for _ in names]
+    t.data = Data(Alignment(names, seqs, 'standard', symbols=symbols))
+    S = len(symbols)
+    t.newComp(free=1, spec='equal')
+    t.newRMatrix(free=1, spec='specified', val=list(rng.lognormal(0, 0.5, S * (S - 1) // 2)))
+    t.setNGammaCat(nGammaCat=3)
+    t.newGdasrv(free=1, val=0.8)
+    t.setPInvar(free=1, val=0.1)
+    want = oracle.part_loglike(oracle.flat_from_tree(t, 0))
+    assert rel(t.calcLogLike(verbose=0), want) < FP64_TOL
+    t.setPrecision("fp32")
+    assert rel(t.calcLogLike(verbose=0), want) < FP32_TOL
+
+
+def test_no_gamma_single_category():
+    tree, _ = synth.make_workload("gtr16", seed=6, nSites=600)
+    mp = tree.model.parts[0]
+    tree.deleteCStuff()
+    mp.gdasrvs = []
+    tree.setNGammaCat(nGammaCat=1)
+    for n in tree.nodes:
+        n.br.parts[0].gdasrvNum = -1
+    tree.setPInvar(free=0, val=0.15)
+    want = oracle.part_loglike(oracle.flat_from_tree(tree, 0))
+    assert rel(tree.calcLogLike(verbose=0), want) < FP64_TOL
