@@ -331,8 +331,7 @@ static bool allocTreePart(Tree *t, int pNum)
     tp.nSlots = ns + 1;
     tp.nEig = mp->nComps * mp->nRMatrices;
     const size_t nPS = (size_t)t->nNodes * K * S * S;
-    const int PB = pruneBlockPatterns(K);
-    tp.maxBlocks = tp.nPatPad / PB;
+    tp.maxBlocks = tp.nPatPad / std::min(pruneBlockPatternsFor(S, K, t->precision), pruneBlockPatterns(K));
     const size_t elt = t->precision == FP32 ? sizeof(float) : sizeof(double);
     tp.clBytes = (size_t)tp.nPatPad * K * S * elt;
     if (!devAlloc(&tp.eigV, (size_t)tp.nEig * S * S, "hipMalloc eigV") ||
@@ -808,9 +807,11 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
     PruneArgs a{};
     fillPruneArgs(t, pNum, nOps, a, siteLikes);
     hipStream_t s = stream();
-    const int nBlocks = tp.nPatPad / pruneBlockPatterns(a.K);
+    const int nBlocks = tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision);
     if (g_timing) (void)hipEventRecord(tp.ev0, s);
-    if (!hipOk(launchPrune(a, t->precision, nBlocks, s), "launch pruning kernel")) return false;
+    const hipError_t le = pruneUsesMfma(a.dim, a.K, t->precision) ? launchPruneAa(a, nBlocks, s)
+                                                                   : launchPrune(a, t->precision, nBlocks, s);
+    if (!hipOk(le, "launch pruning kernel")) return false;
     if (g_timing) (void)hipEventRecord(tp.ev1, s);
     if (!hipOk(hipMemcpyAsync(tp.hResult, tp.result, 2 * sizeof(double), hipMemcpyDeviceToHost, s),
                "copy result"))
@@ -1713,10 +1714,13 @@ int p4g_getCondLikes(const p4g_node *n, int pNum, double *out)
         for (size_t i = 0; i < per; i++) v[i] = f[i];
     } else if (!hipOk(hipMemcpy(v.data(), (const char *)tp.cl + tp.clBytes * sl, per * 8, hipMemcpyDeviceToHost), "copy"))
         return 1;
+    const bool tiled = pruneUsesMfma(S, K, t->precision);
     for (int k = 0; k < K; k++)
         for (int s = 0; s < S; s++)
             for (int p = 0; p < nP; p++) {
-                double x = v[((size_t)p * K + k) * S + s];
+                // tile-blocked layout of the MFMA kernel: [tile][k][s/4][s%4][p%16]
+                double x = tiled ? v[(size_t)(p / 16) * K * S * 16 + ((size_t)(k * 5 + s / 4) * 4 + s % 4) * 16 + p % 16]
+                                 : v[((size_t)p * K + k) * S + s];
                 if (!e.empty()) x = std::ldexp(x, -e[p]);
                 out[((size_t)k * S + s) * nP + p] = x;
             }
@@ -1782,7 +1786,11 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     }
     const int nItems = (int)items.size();
     int nBlocksTot = 0;
-    for (auto &it : items) nBlocksTot += it.t->tp[it.p].nPatPad / pruneBlockPatterns(it.t->model->parts[it.p]->nCat);
+    auto itemBlocks = [](const Item &it) {
+        const ModelPart *mp = it.t->model->parts[it.p];
+        return it.t->tp[it.p].nPatPad / pruneBlockPatternsFor(mp->dim, mp->nCat, it.t->precision);
+    };
+    for (auto &it : items) nBlocksTot += itemBlocks(it);
     if (nItems > b->capItems) {
         devFree(b->dArgs);
         hostFree(b->hArgs);
@@ -1804,7 +1812,12 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     // group by (dim, precision): items of a group are contiguous in hArgs
     std::vector<int> order(nItems);
     for (int i = 0; i < nItems; i++) order[i] = i;
-    auto key = [&](int i) { return items[i].t->model->parts[items[i].p]->dim * 2 + items[i].t->precision; };
+    auto key = [&](int i) {
+        const ModelPart *mp = items[i].t->model->parts[items[i].p];
+        const int prec = items[i].t->precision;
+        const int kc = pruneUsesMfma(mp->dim, mp->nCat, prec) ? (mp->nCat <= 1 ? 1 : mp->nCat <= 2 ? 2 : mp->nCat <= 4 ? 4 : 8) : 0;
+        return (mp->dim * 2 + prec) * 16 + kc;
+    };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
     for (int j = 0; j < nItems; j++) {
         const Item &it = items[order[j]];
@@ -1818,13 +1831,14 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     std::vector<std::pair<int, int>> launches;  // (first block, count) per group
     std::vector<int> groupDim, groupPrec;
     std::vector<size_t> groupLds;
+    std::vector<int> groupK;
     while (j < nItems) {
         const int k0 = key(order[j]);
         const int first = blk;
         int jj = j;
         while (jj < nItems && key(order[jj]) == k0) {
             const Item &it = items[order[jj]];
-            const int nb = it.t->tp[it.p].nPatPad / pruneBlockPatterns(it.t->model->parts[it.p]->nCat);
+            const int nb = itemBlocks(it);
             for (int q = 0; q < nb; q++) {
                 b->hBlockItem[2 * blk] = jj;
                 b->hBlockItem[2 * blk + 1] = q;
@@ -1835,6 +1849,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         launches.push_back({first, blk - first});
         groupDim.push_back(items[order[j]].t->model->parts[items[order[j]].p]->dim);
         groupPrec.push_back(items[order[j]].t->precision);
+        groupK.push_back(items[order[j]].t->model->parts[items[order[j]].p]->nCat);
         // P in LDS for the whole launch only if every item's P rows fit
         size_t lds = 0;
         bool fits = true;
@@ -1850,11 +1865,14 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     if (!hipOk(hipMemcpyAsync(b->dBlockItem, b->hBlockItem, sizeof(int) * 2 * blk, hipMemcpyHostToDevice, s),
                "stage batch map"))
         return 1;
-    for (size_t g = 0; g < launches.size(); g++)
-        if (!hipOk(launchPruneBatch(b->dArgs, b->dBlockItem + 2 * launches[g].first, launches[g].second,
-                                    groupDim[g], groupPrec[g], groupLds[g], s),
-                   "launch batched pruning kernel"))
-            return 1;
+    for (size_t g = 0; g < launches.size(); g++) {
+        const int *bi = b->dBlockItem + 2 * launches[g].first;
+        const hipError_t le = pruneUsesMfma(groupDim[g], groupK[g], groupPrec[g])
+                                  ? launchPruneAaBatch(b->dArgs, bi, launches[g].second, groupK[g], s)
+                                  : launchPruneBatch(b->dArgs, bi, launches[g].second, groupDim[g], groupPrec[g],
+                                                     groupLds[g], s);
+        if (!hipOk(le, "launch batched pruning kernel")) return 1;
+    }
     for (int q = 0; q < nItems; q++) {
         const Item &it = items[order[q]];
         if (!hipOk(hipMemcpyAsync(b->hRes + 2 * q, it.t->tp[it.p].result, 2 * sizeof(double),

@@ -271,6 +271,11 @@ hipError_t launchPruneBatch(const PruneArgs *dArgs, const int *dBlockItem, int n
 int pruneBlockPatterns(int K);  // patterns per 256-thread block
 int pruneGroupWidth(int dim);   // ops per binary group
 size_t pruneLdsBytes(int nKids, int K, int dim, int precision);  // 0 if P stays in global
+// 20-state FP32 parts use the MFMA kernel and a tile-blocked partial layout
+bool pruneUsesMfma(int dim, int K, int precision);
+int pruneBlockPatternsFor(int dim, int K, int precision);
+hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s);
+hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, hipStream_t s);
 
 // Host numerics (p4g_host.hip).
 int discreteGamma(double *freqK, double *rK, double alfa, double beta, int K, int median);

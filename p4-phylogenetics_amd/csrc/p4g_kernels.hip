@@ -687,4 +687,216 @@ hipError_t launchPruneBatch(const PruneArgs *dArgs, const int *dBlockItem, int n
     }
 }
 
+// ------------------------------------------------ 20-state pruning on MFMA
+// One wavefront owns a tile of 16 site patterns (all gamma categories); a
+// 256-thread workgroup owns 4 tiles.  Partials of a 20-state FP32 part are
+// tile-blocked, [slot][tile][cat][t = state/4][state%4][pattern%16], so the
+// B operand of v_mfma_f32_16x16x4f32 for k-step t is 64 consecutive floats.
+// Per child and category: out[s][p] = sum_j P[s][j] x[j][p] as 5 k-steps
+// of 4 on two 16-row tiles (states 0-15 and 16-19; rows 20-31 are zero
+// padding), with A[s][j] = P^T[j][s] read from the P^T table.
+// Lane l holds A[l%16][l/16], B[l/16][l%16] and D rows 4*(l/16)+r, r<4,
+// of column l%16.
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr int AA_S = 20;
+constexpr int AA_TILE = 16;
+
+__device__ __forceinline__ f4v mfma4(float a, float b, f4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <int KC>
+__device__ __forceinline__ void pruneAaBody(const PruneArgs &a, int blk, int nBlk)
+{
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int q = lane >> 4, pl = lane & 15;
+    const int K = a.K;
+    const int tile = blk * (PRUNE_THREADS / 64) + wave;
+    const int p = tile * AA_TILE + pl;
+    const size_t nPatPad = (size_t)a.nPatPad;
+    const size_t tileFloats = (size_t)K * AA_S * AA_TILE;
+    const size_t slotFloats = nPatPad * K * AA_S;
+    const float *PT = (const float *)a.PT;
+    float *cl = (float *)a.cl;
+    const f4v zero = {0.f, 0.f, 0.f, 0.f};
+    const f4v ones = {1.f, 1.f, 1.f, 1.f};
+
+    double contrib = 0.0;
+    int bad = 0;
+    for (int g = 0; g < a.nGroups; ++g) {
+        const GroupRec gr = a.groups[g];
+        for (int o = 0; o < gr.nOps; ++o) {
+            const OpRec op = a.ops[gr.opStart + o];
+            f4v pr0[KC], pr1[KC];
+            int e = 0;
+            for (int c = 0; c < op.nKids; ++c) {
+                const KidRec kd = a.kids[op.kidStart + c];
+                const float *Pn = PT + (size_t)kd.pnode * K * AA_S * AA_S;
+                f4v v0[KC], v1[KC];
+                bool skip = false;
+                if (kd.kind == 0) {
+                    const unsigned code = a.tips[(size_t)kd.idx * nPatPad + p];
+                    skip = (code == TIP_ALL) && c != 0;
+#pragma unroll
+                    for (int k = 0; k < KC; k++) {
+                        if (k >= K) break;
+                        const float *Pk = Pn + k * AA_S * AA_S;
+                        if (code < (unsigned)AA_S) {
+                            v0[k] = *reinterpret_cast<const f4v *>(Pk + code * AA_S + 4 * q);
+                            v1[k] = q == 0 ? *reinterpret_cast<const f4v *>(Pk + code * AA_S + 16) : zero;
+                        } else if (code == TIP_ALL) {
+                            v0[k] = ones;
+                            v1[k] = q == 0 ? ones : zero;
+                        } else {
+                            const uint64_t m = a.ambMask[code - AA_S];
+                            v0[k] = zero;
+                            v1[k] = zero;
+                            for (int j = 0; j < AA_S; j++)
+                                if ((m >> j) & 1ull) {
+                                    v0[k] += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 4 * q);
+                                    if (q == 0) v1[k] += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 16);
+                                }
+                        }
+                    }
+                } else {
+                    const float *X = cl + (size_t)kd.idx * slotFloats + (size_t)tile * tileFloats;
+#pragma unroll
+                    for (int k = 0; k < KC; k++) {
+                        if (k >= K) break;
+                        const float *Pk = Pn + k * AA_S * AA_S;
+                        f4v acc0 = zero, acc1 = zero;
+#pragma unroll
+                        for (int t = 0; t < 5; t++) {
+                            const float b = X[(k * 5 + t) * 64 + lane];
+                            const float a0 = Pk[(4 * t + q) * AA_S + pl];
+                            const float a1 = pl < 4 ? Pk[(4 * t + q) * AA_S + 16 + pl] : 0.f;
+                            acc0 = mfma4(a0, b, acc0);
+                            acc1 = mfma4(a1, b, acc1);
+                        }
+                        v0[k] = acc0;
+                        v1[k] = acc1;
+                    }
+                    e += a.scl[(size_t)kd.idx * nPatPad + p];
+                }
+#pragma unroll
+                for (int k = 0; k < KC; k++) {
+                    if (k >= K) break;
+                    if (c == 0) {
+                        pr0[k] = v0[k];
+                        pr1[k] = v1[k];
+                    } else if (!skip) {
+                        pr0[k] *= v0[k];
+                        pr1[k] *= v1[k];
+                    }
+                }
+            }
+            // rescale: max over categories and states of this pattern
+            float m = 0.f;
+#pragma unroll
+            for (int k = 0; k < KC; k++) {
+                if (k >= K) break;
+                for (int r = 0; r < 4; r++) m = fmaxf(m, fmaxf(pr0[k][r], pr1[k][r]));
+            }
+            m = fmaxf(m, __shfl_xor(m, 16));
+            m = fmaxf(m, __shfl_xor(m, 32));
+            if (m < SCALE_THRESH && m > 0.f) {
+                int ex;
+                frexpf(m, &ex);
+                const int E = -ex;
+#pragma unroll
+                for (int k = 0; k < KC; k++) {
+                    if (k >= K) break;
+                    for (int r = 0; r < 4; r++) {
+                        pr0[k][r] = ldexpf(pr0[k][r], E);
+                        pr1[k][r] = ldexpf(pr1[k][r], E);
+                    }
+                }
+                e += E;
+            }
+            if (!(op.flags & OP_ROOT)) {
+                float *D = cl + (size_t)op.dst * slotFloats + (size_t)tile * tileFloats;
+#pragma unroll
+                for (int k = 0; k < KC; k++) {
+                    if (k >= K) break;
+                    for (int r = 0; r < 4; r++) D[((k * 5 + q) * 4 + r) * AA_TILE + pl] = pr0[k][r];
+                    if (q == 0)
+                        for (int r = 0; r < 4; r++) D[((k * 5 + 4) * 4 + r) * AA_TILE + pl] = pr1[k][r];
+                }
+                if (q == 0) a.scl[(size_t)op.dst * nPatPad + p] = e;
+                continue;
+            }
+            // root: sum_k sum_s pi_s cl_s over this lane's states, then
+            // across the four lanes of the pattern
+            const double *pi = a.rootPi;
+            uint64_t msk = ~0ull;
+            if (a.rootLeafRow >= 0) {
+                const unsigned code = a.tips[(size_t)a.rootLeafRow * nPatPad + p];
+                msk = code < (unsigned)AA_S ? (1ull << code) : code == TIP_ALL ? ~0ull : a.ambMask[code - AA_S];
+            }
+            double Lk = 0.0;
+#pragma unroll
+            for (int k = 0; k < KC; k++) {
+                if (k >= K) break;
+                for (int r = 0; r < 4; r++) {
+                    const int s0 = 4 * q + r;
+                    if ((msk >> s0) & 1ull) Lk += pi[s0] * (double)pr0[k][r];
+                    if (q == 0 && ((msk >> (16 + r)) & 1ull)) Lk += pi[16 + r] * (double)pr1[k][r];
+                }
+            }
+            Lk += __shfl_xor(Lk, 16);
+            Lk += __shfl_xor(Lk, 32);
+            if (q == 0) siteLogLike<true>(a, Lk, e, p, K, pi, contrib, bad);
+        }
+    }
+    finishSiteSum(a, contrib, bad, blk, nBlk);
+}
+
+template <int KC>
+__global__ __launch_bounds__(PRUNE_THREADS) void k_pruneAa(PruneArgs a)
+{
+    pruneAaBody<KC>(a, blockIdx.x, gridDim.x);
+}
+
+template <int KC>
+__global__ __launch_bounds__(PRUNE_THREADS) void k_pruneAaBatch(const PruneArgs *args, const int *blockItem)
+{
+    const int item = blockItem[2 * blockIdx.x];
+    const int local = blockItem[2 * blockIdx.x + 1];
+    const PruneArgs a = args[item];
+    pruneAaBody<KC>(a, local, a.nPatPad / (AA_TILE * PRUNE_THREADS / 64));
+}
+
+bool pruneUsesMfma(int dim, int K, int precision) { return dim == AA_S && precision == FP32 && K <= 8; }
+
+int pruneBlockPatternsFor(int dim, int K, int precision)
+{
+    return pruneUsesMfma(dim, K, precision) ? AA_TILE * PRUNE_THREADS / 64 : pruneBlockPatterns(K);
+}
+
+static int aaKC(int K) { return K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : 8; }
+
+hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s)
+{
+    switch (aaKC(a.K)) {
+    case 1: hipLaunchKernelGGL(k_pruneAa<1>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a); break;
+    case 2: hipLaunchKernelGGL(k_pruneAa<2>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(k_pruneAa<4>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a); break;
+    default: hipLaunchKernelGGL(k_pruneAa<8>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, hipStream_t s)
+{
+    switch (aaKC(K)) {
+    case 1: hipLaunchKernelGGL(k_pruneAaBatch<1>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi); break;
+    case 2: hipLaunchKernelGGL(k_pruneAaBatch<2>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi); break;
+    case 4: hipLaunchKernelGGL(k_pruneAaBatch<4>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi); break;
+    default: hipLaunchKernelGGL(k_pruneAaBatch<8>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi); break;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace p4g

@@ -243,8 +243,11 @@ class Chain(object):
     def sync(self, src, dst):
         """Make dst's Python and GPU state equal to src's after a
         proposal (p4/chain.py:1351-1531)."""
-        for n in self.touched:
-            m = dst.nodes[n.nodeNum]
+        # the touched set names nodes by number; src and dst have the same
+        # node numbering
+        for num in set(n.nodeNum for n in self.touched):
+            n = src.nodes[num]
+            m = dst.nodes[num]
             m.br.len = n.br.len
             m.parent = dst.nodes[n.parent.nodeNum] if n.parent else None
             m.leftChild = dst.nodes[n.leftChild.nodeNum] if n.leftChild else None
