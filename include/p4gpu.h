@@ -221,7 +221,36 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL);
  * p4g_lastKernelMicroseconds returns the device time of the last
  * evaluation's pruning launches of tree t (summed over parts), -1 if none. */
 void p4g_setKernelTiming(int on);
+/* Tuning of the 4-state pruning kernel (no counterpart in the reference;
+ * set by the autotuner, p4gpu/autotune.py): independent binary ops per
+ * group (1, 2, 4 or 8) and whether the program's P rows are staged in LDS,
+ * for FP64 (fp32 = 0) or FP32 partials.  Results do not depend on it. */
+void p4g_setPruneTuning(int fp32, int opsPerGroup, int stagePInLds);
 double p4g_lastKernelMicroseconds(const p4g_tree *t);
+
+/* ---- learned proposal network (no counterpart in the reference, whose
+ * proposals are all hand-designed, p4/mcmc.py:1600-2200).  A 3-layer MLP
+ * nIn -> nHidden -> nHidden -> nOut (ReLU) with BF16 weights on the MFMA
+ * units and FP32 accumulation / master weights / Adam.  nOut = 2D: columns
+ * [0, D) are Gaussian means, [D, 2D) log standard deviations.
+ * x: [n][nIn] row-major; y, mask: [n][D]; out: [n][nOut]; grads: the
+ * parameter vector layout W1 [H][F], b1 [H], W2 [H][H], b2 [H], W3 [O][H],
+ * b3 [O].  loss = sum_i sum_d mask (s + (y - mu)^2 e^{-2s} / 2) over the n
+ * rows (mu, s: the row's mean and log sd); grads = d(loss / n)/dparams. */
+typedef struct p4g_mlp p4g_mlp;
+p4g_mlp *p4g_mlpNew(int nIn, int nHidden, int nOut, unsigned long long seed);
+void p4g_mlpFree(p4g_mlp *m);
+int p4g_mlpNumParams(const p4g_mlp *m);
+int p4g_mlpGetParams(p4g_mlp *m, float *out);
+int p4g_mlpSetParams(p4g_mlp *m, const float *in);
+int p4g_mlpForward(p4g_mlp *m, const float *x, int n, float *out);
+/* Loss and gradients without an update. */
+int p4g_mlpGradients(p4g_mlp *m, const float *x, const float *y, const float *mask, int n, float *grads,
+                     float *loss);
+/* One Adam step (b1 0.9, b2 0.999, eps 1e-8) on grads at learning rate lr;
+ * loss is the batch loss before the step. */
+int p4g_mlpTrainStep(p4g_mlp *m, const float *x, const float *y, const float *mask, int n, float lr,
+                     float *loss);
 
 #ifdef __cplusplus
 }

@@ -135,17 +135,10 @@ static void freeTreePart(TreePart &t)
     devFree(t.counter);
     devFree(t.result);
     devFree(t.patLikes);
-    devFree(t.prog);
-    devFree(t.rootPi);
-    devFree(t.dEigIn);
-    devFree(t.dPack);
-    hostFree(t.hEigIn);
-    hostFree(t.hEigList);
+    devFree(t.dArena);
+    hostFree(t.hArena);
     hostFree(t.hT);
     hostFree(t.hEs);
-    hostFree(t.hPack);
-    hostFree(t.hRootPi);
-    hostFree(t.hProg);
     hostFree(t.hResult);
     if (t.ev0) (void)hipEventDestroy(t.ev0);
     if (t.ev1) (void)hipEventDestroy(t.ev1);
@@ -341,31 +334,32 @@ static bool allocTreePart(Tree *t, int pNum)
         !devAlloc(&tp.blockSums, tp.maxBlocks, "hipMalloc blockSums") ||
         !devAlloc(&tp.blockBad, tp.maxBlocks, "hipMalloc blockBad") ||
         !devAlloc(&tp.counter, 1, "hipMalloc counter") || !devAlloc(&tp.result, 2, "hipMalloc result") ||
-        !devAlloc(&tp.patLikes, tp.nPatPad, "hipMalloc patLikes") ||
-        !devAlloc(&tp.rootPi, MAX_DIM, "hipMalloc rootPi"))
+        !devAlloc(&tp.patLikes, tp.nPatPad, "hipMalloc patLikes"))
         return false;
     if (!tp.cl && !hipOk(hipMalloc(&tp.cl, tp.clBytes * tp.nSlots), "hipMalloc partials")) return false;
     if (t->precision == FP32 && !devAlloc(&tp.scl, (size_t)tp.nSlots * tp.nPatPad, "hipMalloc scale"))
         return false;
-    if (!tp.prog) {
-        tp.progCap = 8 * (2 * t->nNodes + 4);
-        if (!devAlloc(&tp.prog, tp.progCap, "hipMalloc program")) return false;
-        if (!hostAlloc(&tp.hProg, tp.progCap, "hipHostMalloc program")) return false;
-        tp.hProgCap = tp.progCap;
-    }
-    if (!tp.hEigIn) {
-        const size_t eigWords = (size_t)tp.nEig * (S + S * S);
+    if (!tp.hArena) {
+        auto up16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+        const size_t eigBytes = up16(sizeof(double) * (size_t)tp.nEig * (S + S * S));
         tp.packBytes = (size_t)t->nNodes * (K * sizeof(double) + 2 * sizeof(int));
-        if (!hostAlloc(&tp.hEigIn, eigWords, "hipHostMalloc eigen inputs") ||
-            !hostAlloc(&tp.hEigList, tp.nEig, "hipHostMalloc eigen list") ||
-            !devAlloc(&tp.dEigIn, eigWords + tp.nEig, "hipMalloc eigen inputs") ||
+        tp.offEigList = eigBytes;
+        tp.offPack = tp.offEigList + up16(sizeof(int) * tp.nEig);
+        tp.offPi = tp.offPack + up16(tp.packBytes);
+        tp.offProg = tp.offPi + up16(sizeof(double) * MAX_DIM);
+        tp.progCapWords = 16 * t->nNodes + 64;
+        tp.arenaBytes = tp.offProg + sizeof(int) * tp.progCapWords;
+        if (!hostAlloc(&tp.hArena, tp.arenaBytes, "hipHostMalloc staging arena") ||
+            !devAlloc(&tp.dArena, tp.arenaBytes, "hipMalloc staging arena") ||
             !hostAlloc(&tp.hT, (size_t)t->nNodes * K, "hipHostMalloc branch lengths") ||
             !hostAlloc(&tp.hEs, t->nNodes, "hipHostMalloc eigen slots") ||
-            !hostAlloc(&tp.hPack, tp.packBytes, "hipHostMalloc P items") ||
-            !devAlloc(&tp.dPack, tp.packBytes, "hipMalloc P items") ||
-            !hostAlloc(&tp.hRootPi, MAX_DIM, "hipHostMalloc root comp") ||
             !hostAlloc(&tp.hResult, 2, "hipHostMalloc result"))
             return false;
+        tp.hEigIn = reinterpret_cast<double *>(tp.hArena);
+        tp.hEigList = reinterpret_cast<int *>(tp.hArena + tp.offEigList);
+        tp.hPack = tp.hArena + tp.offPack;
+        tp.hRootPi = reinterpret_cast<double *>(tp.hArena + tp.offPi);
+        tp.hProg = reinterpret_cast<int *>(tp.hArena + tp.offProg);
         if (!hipOk(hipEventCreate(&tp.ev0), "hipEventCreate") ||
             !hipOk(hipEventCreate(&tp.ev1), "hipEventCreate"))
             return false;
@@ -460,56 +454,73 @@ static void markP(Tree *t, int pNum, const Node *nd)
     tp.pValid[n] = 1;
 }
 
-// Enqueue the pending eigensystems, then the pending P matrices, of one
-// part (one launch each) on the library stream.
-static bool flushParams(Tree *t, int pNum)
+// Pack the pending eigensystem slots and P items of one part into the
+// staging arena (no copies, no launches).
+static void stageParams(Tree *t, int pNum)
 {
     TreePart &tp = t->tp[pNum];
-    ModelPart *mp = t->model->parts[pNum];
-    const int S = mp->dim, K = mp->nCat;
-    hipStream_t s = stream();
+    const int K = t->model->parts[pNum]->nCat;
     int nE = 0;
     for (int i = 0; i < tp.nEig; i++)
         if (tp.eigPending[i]) {
             tp.hEigList[nE++] = i;
             tp.eigPending[i] = 0;
         }
-    if (nE) {
-        const size_t words = (size_t)tp.nEig * (S + S * S);
-        if (!hipOk(hipMemcpyAsync(tp.dEigIn, tp.hEigIn, sizeof(double) * words, hipMemcpyHostToDevice, s),
-                   "stage eigen inputs") ||
-            !hipOk(hipMemcpyAsync(tp.dEigIn + words, tp.hEigList, sizeof(int) * nE, hipMemcpyHostToDevice, s),
-                   "stage eigen list"))
-            return false;
-        EigArgs a{S, nE, reinterpret_cast<const int *>(tp.dEigIn + words), tp.dEigIn, tp.eigV, tp.eigVi,
-                  tp.eigLam};
-        if (!hipOk(launchEig(a, s), "launch eigensystem kernel")) return false;
-    }
     int nI = 0;
     for (int n = 0; n < t->nNodes; n++) nI += tp.pPending[n] ? 1 : 0;
-    if (nI) {
-        double *ht = reinterpret_cast<double *>(tp.hPack);
-        int *hn = reinterpret_cast<int *>(ht + (size_t)nI * K);
-        int *he = hn + nI;
-        int i = 0;
-        for (int n = 0; n < t->nNodes; n++) {
-            if (!tp.pPending[n]) continue;
-            for (int k = 0; k < K; k++) ht[(size_t)i * K + k] = tp.hT[(size_t)n * K + k];
-            hn[i] = n;
-            he[i] = tp.hEs[n];
-            tp.pPending[n] = 0;
-            i++;
-        }
-        const size_t bytes = sizeof(double) * nI * K + sizeof(int) * 2 * nI;
-        if (!hipOk(hipMemcpyAsync(tp.dPack, tp.hPack, bytes, hipMemcpyHostToDevice, s), "stage P inputs"))
-            return false;
-        const char *d = tp.dPack;
+    double *ht = reinterpret_cast<double *>(tp.hPack);
+    int *hn = reinterpret_cast<int *>(ht + (size_t)nI * K);
+    int *he = hn + nI;
+    int i = 0;
+    for (int n = 0; n < t->nNodes && nI; n++) {
+        if (!tp.pPending[n]) continue;
+        for (int k = 0; k < K; k++) ht[(size_t)i * K + k] = tp.hT[(size_t)n * K + k];
+        hn[i] = n;
+        he[i] = tp.hEs[n];
+        tp.pPending[n] = 0;
+        i++;
+    }
+    tp.pendE = nE;
+    tp.pendI = nI;
+}
+
+// One host-to-device copy of the arena prefix [0, bytes).
+static bool uploadArena(TreePart &tp, size_t bytes)
+{
+    return hipOk(hipMemcpyAsync(tp.dArena, tp.hArena, bytes, hipMemcpyHostToDevice, stream()), "stage inputs");
+}
+
+// Launch the staged eigensystems, then the staged P matrices (they read the
+// uploaded arena).
+static bool launchParams(Tree *t, int pNum)
+{
+    TreePart &tp = t->tp[pNum];
+    ModelPart *mp = t->model->parts[pNum];
+    const int S = mp->dim, K = mp->nCat;
+    hipStream_t s = stream();
+    if (tp.pendE) {
+        EigArgs a{S, tp.pendE, reinterpret_cast<const int *>(tp.dArena + tp.offEigList),
+                  reinterpret_cast<const double *>(tp.dArena), tp.eigV, tp.eigVi, tp.eigLam};
+        if (!hipOk(launchEig(a, s), "launch eigensystem kernel")) return false;
+    }
+    if (tp.pendI) {
+        const int nI = tp.pendI;
+        const char *d = tp.dArena + tp.offPack;
         PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
                    reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
                    reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
         if (!hipOk(launchPmat(a, s), "launch P kernel")) return false;
     }
+    tp.pendE = tp.pendI = 0;
     return true;
+}
+
+static bool flushParams(Tree *t, int pNum)
+{
+    stageParams(t, pNum);
+    TreePart &tp = t->tp[pNum];
+    if (!tp.pendE && !tp.pendI) return true;
+    return uploadArena(tp, tp.offPi) && launchParams(t, pNum);
 }
 
 // Flush every part and wait (for inspection and copy entry points).
@@ -670,7 +681,7 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
     }
     // Group ops by height above the nodes already computed: ops of equal
     // height are independent, so binary ones are packed G to a group.
-    const int G = pruneGroupWidth(t->model->parts[pNum]->dim);
+    const int G = pruneGroupWidthFor(t->model->parts[pNum]->dim, t->precision);
     const int nOps = (int)ops.size();
     std::vector<int> opOfSlot(tp.nSlots, -1), height(nOps, 0);
     for (int i = 0; i < nOps; i++) {
@@ -710,19 +721,16 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
         q = r;
     }
     const size_t words = groups.size() * 4 + ops2.size() * 4 + kids2.size() * 4;
-    if ((int)words > tp.hProgCap) {
-        hostFree(tp.hProg);
-        devFree(tp.prog);
-        tp.progCap = tp.hProgCap = (int)words * 2;
-        if (!devAlloc(&tp.prog, tp.progCap, "hipMalloc program") ||
-            !hostAlloc(&tp.hProg, tp.hProgCap, "hipHostMalloc program"))
-            return false;
+    if ((int)words > tp.progCapWords) {
+        setError("internal: node program larger than its staging area");
+        return false;
     }
     std::memcpy(tp.hProg, groups.data(), groups.size() * sizeof(GroupRec));
     std::memcpy(tp.hProg + groups.size() * 4, ops2.data(), ops2.size() * sizeof(OpRec));
     std::memcpy(tp.hProg + (groups.size() + ops2.size()) * 4, kids2.data(), kids2.size() * sizeof(KidRec));
     tp.nGroups = (int)groups.size();
     tp.nKids = (int)kids2.size();
+    tp.progWords = (int)words;
     *nOpsOut = (int)ops2.size();
     *nKidsOut = (int)kids2.size();
     return true;
@@ -734,9 +742,10 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     ModelPart *mp = t->model->parts[pNum];
     Part *dp = t->data->parts[pNum];
     const Node *root = t->nodes[t->root];
-    a.groups = reinterpret_cast<const GroupRec *>(tp.prog);
-    a.ops = reinterpret_cast<const OpRec *>(tp.prog + tp.nGroups * 4);
-    a.kids = reinterpret_cast<const KidRec *>(tp.prog + (tp.nGroups + nOps) * 4);
+    const int *prog = reinterpret_cast<const int *>(tp.dArena + tp.offProg);
+    a.groups = reinterpret_cast<const GroupRec *>(prog);
+    a.ops = reinterpret_cast<const OpRec *>(prog + tp.nGroups * 4);
+    a.kids = reinterpret_cast<const KidRec *>(prog + (tp.nGroups + nOps) * 4);
     a.nGroups = tp.nGroups;
     a.nOps = nOps;
     a.nKids = tp.nKids;
@@ -753,7 +762,7 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     a.invMask = dp->dev.invMask;
     a.nWords = dp->dev.nWords;
     a.ambMask = dp->dev.ambMask;
-    a.rootPi = tp.rootPi;
+    a.rootPi = reinterpret_cast<const double *>(tp.dArena + tp.offPi);
     a.pInvar = mp->pInvar;
     a.rootLeafRow = root->isLeaf ? root->seqNum : -1;
     a.blockSums = tp.blockSums;
@@ -776,15 +785,12 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps)
         setError("root comp out of range");
         return false;
     }
-    double *hpi = tp.hRootPi;
-    std::memcpy(hpi, mp->comps[rc].val, sizeof(double) * mp->dim);
-    hipStream_t s = stream();
-    if (!flushParams(t, pNum)) return false;
-    return hipOk(hipMemcpyAsync(tp.prog, tp.hProg, sizeof(int) * 4 * (tp.nGroups + *nOps + nKids),
-                                hipMemcpyHostToDevice, s),
-                 "stage program") &&
-           hipOk(hipMemcpyAsync(tp.rootPi, hpi, sizeof(double) * mp->dim, hipMemcpyHostToDevice, s),
-                 "stage root comp");
+    std::memcpy(tp.hRootPi, mp->comps[rc].val, sizeof(double) * mp->dim);
+    // pending eigensystems and P, the program and the root comp: one upload,
+    // then the parameter kernels; the pruning launch follows on the stream
+    stageParams(t, pNum);
+    (void)nKids;
+    return uploadArena(tp, tp.offProg + sizeof(int) * tp.progWords) && launchParams(t, pNum);
 }
 
 static void finishPart(Tree *t, int pNum, bool siteLikes, double lnL, bool bad)
@@ -855,6 +861,11 @@ const char *p4g_lastError(void) { return g_err.c_str(); }
 int p4g_deviceAvailable(void) { return deviceReady() ? 1 : 0; }
 
 void p4g_setKernelTiming(int on) { g_timing = on != 0; }
+
+void p4g_setPruneTuning(int fp32, int opsPerGroup, int stagePInLds)
+{
+    setPruneTuning(fp32 ? FP32 : FP64, opsPerGroup, stagePInLds);
+}
 
 double p4g_lastKernelMicroseconds(const p4g_tree *t)
 {

@@ -171,11 +171,8 @@ struct TreePart {
     double *result = nullptr;      // [2] lnL, bad flag
     double *patLikes = nullptr;    // [nPatPad]
     int maxBlocks = 0;
-    // Programs.
-    int *prog = nullptr;           // device groups+ops+kids
-    int progCap = 0;
-    int nGroups = 0, nKids = 0;    // of the staged program
-    double *rootPi = nullptr;      // [dim]
+    // Staged program (device pointers into the arena).
+    int nGroups = 0, nKids = 0, progWords = 0;
     // Host-side flags.
     std::vector<char> dirty;       // per node
     std::vector<char> pValid;      // per node
@@ -184,19 +181,22 @@ struct TreePart {
     // inputs here (pinned host memory) and mark them pending; the pending
     // eigensystems and P matrices are built by one launch each right before
     // the next pruning launch, on the same stream.
-    double *hEigIn = nullptr;      // [nEig][dim + dim*dim]  pi, R
-    int *hEigList = nullptr;       // [nEig] pending slots (packed at flush)
-    double *dEigIn = nullptr;      // device copy of hEigIn + slot list
+    // All per-evaluation host->device staging lives in one pinned arena with
+    // a device mirror at the same offsets, uploaded by one copy.
+    char *hArena = nullptr, *dArena = nullptr;
+    size_t arenaBytes = 0, offEigList = 0, offPack = 0, offPi = 0, offProg = 0;
+    int progCapWords = 0;
+    double *hEigIn = nullptr;      // [nEig][dim + dim*dim]  pi, R   (arena)
+    int *hEigList = nullptr;       // [nEig] pending slots           (arena)
+    char *hPack = nullptr;         // packed pending P items          (arena)
+    double *hRootPi = nullptr;     // [MAX_DIM]                       (arena)
+    int *hProg = nullptr;          // groups, ops, kids               (arena)
     std::vector<char> eigPending;  // per eigen slot
     double *hT = nullptr;          // [nNodes][K] effective branch lengths
     int *hEs = nullptr;            // [nNodes] eigen slot of each node
-    char *hPack = nullptr;         // packed pending P items (t, node, slot)
-    char *dPack = nullptr;
     size_t packBytes = 0;
     std::vector<char> pPending;    // per node
-    double *hRootPi = nullptr;     // [MAX_DIM]
-    int *hProg = nullptr;
-    int hProgCap = 0;
+    int pendE = 0, pendI = 0;      // staged counts for the next launch
     double *hResult = nullptr;     // pinned [2]
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double lastKernelUs = -1.0;
@@ -269,7 +269,8 @@ hipError_t launchPrune(const PruneArgs &a, int precision, int nBlocks, hipStream
 hipError_t launchPruneBatch(const PruneArgs *dArgs, const int *dBlockItem, int nBlocks,
                             int dim, int precision, size_t lds, hipStream_t s);
 int pruneBlockPatterns(int K);  // patterns per 256-thread block
-int pruneGroupWidth(int dim);   // ops per binary group
+int pruneGroupWidthFor(int dim, int precision);  // ops per binary group
+void setPruneTuning(int precision, int G, int lds);
 size_t pruneLdsBytes(int nKids, int K, int dim, int precision);  // 0 if P stays in global
 // 20-state FP32 parts use the MFMA kernel and a tile-blocked partial layout
 bool pruneUsesMfma(int dim, int K, int precision);

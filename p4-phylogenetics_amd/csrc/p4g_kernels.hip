@@ -94,7 +94,7 @@ __global__ __launch_bounds__(64) void k_eig(EigArgs a)
     }
     __syncthreads();
 
-    for (int sweep = 0; sweep < 60; sweep++) {
+    for (int sweep = 0; sweep < 30; sweep++) {
         double off = 0.0, dg = 0.0;
         for (int idx = lane; idx < S * S; idx += 64) {
             int i = idx / S, j = idx % S;
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(64) void k_eig(EigArgs a)
         }
         off = waveSum(off);
         dg = waveSum(dg);
-        if (off == 0.0 || off < 1e-34 * dg) break;
+        if (off == 0.0 || off < 1e-30 * dg) break;  // off-diagonal norm below 1e-15 of the diagonal
         for (int r = 0; r < n - 1; r++) {
             if (lane < h) {
                 int p = rrPlayer(lane, r, n), q = rrPlayer(n - 1 - lane, r, n);
@@ -230,9 +230,6 @@ int pruneBlockPatterns(int K)
     return PRUNE_THREADS / Kp;
 }
 
-// Independent binary ops processed together (their loads are issued before
-// any of their arithmetic): more loads in flight per lane.
-int pruneGroupWidth(int dim) { return dim <= 4 ? 4 : 1; }
 
 template <typename T>
 __device__ __forceinline__ T groupMax(T v, int Kp)
@@ -457,12 +454,28 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
     T *cl = (T *)a.cl;
 
     if (PLDS) {
-        // P^T of every child branch the program uses, [kid][K][S][S]
-        const int per = K * S * S;
-        const int total = a.nKids * per;
-        for (int i = tid; i < total; i += PRUNE_THREADS) {
-            const int kid = i / per;
-            sP[i] = PTg[(size_t)a.kids[kid].pnode * per + (i - kid * per)];
+        // P^T of every child branch the program uses, [kid][K][S][S], copied
+        // in 16-byte chunks with 8 independent loads in flight per thread
+        constexpr int CH = 16 / sizeof(T);           // elements per chunk
+        const int per = K * S * S;                   // a multiple of CH for S = 4, 20
+        const int cpk = per / CH;                    // chunks per kid
+        const int total = a.nKids * cpk;
+        for (int c0 = tid; c0 < total; c0 += PRUNE_THREADS * 8) {
+            float4 buf[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = c0 + u * PRUNE_THREADS;
+                if (c < total) {
+                    const int kid = c / cpk;
+                    const float4 *src = reinterpret_cast<const float4 *>(PTg + (size_t)a.kids[kid].pnode * per);
+                    buf[u] = src[c - kid * cpk];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = c0 + u * PRUNE_THREADS;
+                if (c < total) reinterpret_cast<float4 *>(sP)[c] = buf[u];
+            }
         }
         __syncthreads();
     }
@@ -597,23 +610,21 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
     finishSiteSum(a, contrib, bad, blk, nBlk);
 }
 
-template <int SC, typename T, bool SCALE, bool PLDS>
+template <int SC, typename T, bool SCALE, bool PLDS, int G>
 __global__ __launch_bounds__(PRUNE_THREADS) void k_prune(PruneArgs a)
 {
-    extern __shared__ unsigned char smemRaw[];
-    constexpr int G = (SC == 2 || SC == 4) ? 4 : 1;
+    extern __shared__ __align__(16) unsigned char smemRaw[];
     pruneBody<SC, T, SCALE, PLDS, G>(a, blockIdx.x, gridDim.x, reinterpret_cast<T *>(smemRaw));
 }
 
 // Batched variant: several (tree, part) items of the same state count and
 // precision in one launch; blockItem[2b] = item, blockItem[2b+1] = the
 // block's index within the item.
-template <int SC, typename T, bool SCALE, bool PLDS>
+template <int SC, typename T, bool SCALE, bool PLDS, int G>
 __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneBatch(const PruneArgs *args,
                                                               const int *blockItem)
 {
-    extern __shared__ unsigned char smemRaw[];
-    constexpr int G = (SC == 2 || SC == 4) ? 4 : 1;
+    extern __shared__ __align__(16) unsigned char smemRaw[];
     const int item = blockItem[2 * blockIdx.x];
     const int local = blockItem[2 * blockIdx.x + 1];
     const PruneArgs a = args[item];
@@ -621,16 +632,41 @@ __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneBatch(const PruneArgs *a
     pruneBody<SC, T, SCALE, PLDS, G>(a, local, nBlk, reinterpret_cast<T *>(smemRaw));
 }
 
+// Tuning of the 4-state kernel: ops per binary group (1, 2, 4, 8) and
+// whether P rows are staged in LDS.  Other state counts use 1 op per group.
+static int g_tuneG[2] = {4, 4};      // [precision]
+static int g_tuneLds[2] = {1, 1};
+
+void setPruneTuning(int precision, int G, int lds)
+{
+    if (precision < 0 || precision > 1) return;
+    if (G == 1 || G == 2 || G == 4 || G == 8) g_tuneG[precision] = G;
+    g_tuneLds[precision] = lds ? 1 : 0;
+}
+
+int pruneGroupWidthFor(int dim, int precision) { return dim == 4 ? g_tuneG[precision & 1] : 1; }
+
+template <int SC, typename T, bool SCALE, int G>
+static hipError_t launchPruneG(const PruneArgs &a, size_t lds, int nBlocks, hipStream_t s)
+{
+    // lds <= LDS_P_BUDGET (48 KiB), within the default per-workgroup limit
+    if (lds)
+        hipLaunchKernelGGL((k_prune<SC, T, SCALE, true, G>), dim3(nBlocks), dim3(PRUNE_THREADS), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_prune<SC, T, SCALE, false, G>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
 template <int SC, typename T, bool SCALE>
 static hipError_t launchPruneP(const PruneArgs &a, size_t lds, int nBlocks, hipStream_t s)
 {
-    // lds <= LDS_P_BUDGET (48 KiB), within the default per-workgroup limit
-    if (lds) {
-        hipLaunchKernelGGL((k_prune<SC, T, SCALE, true>), dim3(nBlocks), dim3(PRUNE_THREADS), lds, s, a);
-    } else {
-        hipLaunchKernelGGL((k_prune<SC, T, SCALE, false>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a);
+    if (SC != 4) return launchPruneG<SC, T, SCALE, 1>(a, lds, nBlocks, s);
+    switch (pruneGroupWidthFor(4, SCALE ? FP32 : FP64)) {
+    case 1: return launchPruneG<SC, T, SCALE, 1>(a, lds, nBlocks, s);
+    case 2: return launchPruneG<SC, T, SCALE, 2>(a, lds, nBlocks, s);
+    case 8: return launchPruneG<SC, T, SCALE, 8>(a, lds, nBlocks, s);
+    default: return launchPruneG<SC, T, SCALE, 4>(a, lds, nBlocks, s);
     }
-    return hipGetLastError();
 }
 
 template <int SC>
@@ -642,7 +678,10 @@ static hipError_t launchPruneS(const PruneArgs &a, int precision, size_t lds, in
 
 size_t pruneLdsBytes(int nKids, int K, int dim, int precision)
 {
-    const size_t b = (size_t)nKids * K * dim * dim * (precision == FP32 ? 4 : 8);
+    if (!g_tuneLds[precision & 1]) return 0;
+    const size_t per = (size_t)K * dim * dim * (precision == FP32 ? 4 : 8);
+    if (per % 16) return 0;  // staged in 16-byte chunks
+    const size_t b = (size_t)nKids * per;
     return b <= (size_t)LDS_P_BUDGET ? b : 0;
 }
 
@@ -657,15 +696,26 @@ hipError_t launchPrune(const PruneArgs &a, int precision, int nBlocks, hipStream
     }
 }
 
+template <int SC, typename T, bool SCALE, int G>
+static hipError_t launchBatchG(const PruneArgs *d, const int *bi, int nBlocks, size_t lds, hipStream_t s)
+{
+    if (lds)
+        hipLaunchKernelGGL((k_pruneBatch<SC, T, SCALE, true, G>), dim3(nBlocks), dim3(PRUNE_THREADS), lds, s, d, bi);
+    else
+        hipLaunchKernelGGL((k_pruneBatch<SC, T, SCALE, false, G>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi);
+    return hipGetLastError();
+}
+
 template <int SC, typename T, bool SCALE>
 static hipError_t launchBatchP(const PruneArgs *d, const int *bi, int nBlocks, size_t lds, hipStream_t s)
 {
-    if (lds) {
-        hipLaunchKernelGGL((k_pruneBatch<SC, T, SCALE, true>), dim3(nBlocks), dim3(PRUNE_THREADS), lds, s, d, bi);
-    } else {
-        hipLaunchKernelGGL((k_pruneBatch<SC, T, SCALE, false>), dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi);
+    if (SC != 4) return launchBatchG<SC, T, SCALE, 1>(d, bi, nBlocks, lds, s);
+    switch (pruneGroupWidthFor(4, SCALE ? FP32 : FP64)) {
+    case 1: return launchBatchG<SC, T, SCALE, 1>(d, bi, nBlocks, lds, s);
+    case 2: return launchBatchG<SC, T, SCALE, 2>(d, bi, nBlocks, lds, s);
+    case 8: return launchBatchG<SC, T, SCALE, 8>(d, bi, nBlocks, lds, s);
+    default: return launchBatchG<SC, T, SCALE, 4>(d, bi, nBlocks, lds, s);
     }
-    return hipGetLastError();
 }
 
 template <int SC>

@@ -52,7 +52,7 @@ class Proposal(object):
         """Move the tuning toward an acceptance rate inside target
         (p4/mcmc.py:250-283)."""
         n = self.tnNSamples[tempNum]
-        if n < 50:
+        if n < 50 or self.name == 'learned':
             return
         acc = self.tnNAccepts[tempNum] / n
         t = self.tuning[tempNum]
@@ -358,6 +358,8 @@ class Mcmc(object):
         self.learned = None
         self.swapMatrix = numpy.zeros((nChains, nChains), int)
         self.samples = dict(gen=[], lnL=[], treeLen=[], splits=[], prams=[])
+        # callables f(tree) called with the cold chain's tree at every sample
+        self.sampleHooks = []
         self._batch = pf.Batch([ch.propTree.cTree for ch in self.chains])
         self.tuneInterval = 250
         self.timing = dict(seconds=0.0, gens=0)
@@ -458,6 +460,8 @@ class Mcmc(object):
                 pr.append(float(g.val[0]))
             pr.append(mp.pInvar.val)
         self.samples['prams'].append(pr)
+        for f in self.sampleHooks:
+            f(t)
 
     # ------------------------------------------------------------ judges
     def splitFrequencies(self, burnin=0):

@@ -48,7 +48,26 @@ def lib():
                        "Run: python p4-phylogenetics_amd/build.py"])
     _lib = ctypes.CDLL(LIB_PATH)
     _declare()
+    _applyTuning()
     return _lib
+
+
+TUNING_PATH = os.path.join(_HERE, "tuning.json")
+
+
+def _applyTuning():
+    """Apply the autotuner's choices (p4gpu/tuning.json) if present."""
+    import json
+    if not os.path.exists(TUNING_PATH):
+        return
+    try:
+        with open(TUNING_PATH) as fh:
+            tun = json.load(fh)
+        for key, fp32 in (("fp64", 0), ("fp32", 1)):
+            if key in tun:
+                _lib.p4g_setPruneTuning(fp32, int(tun[key]["opsPerGroup"]), int(tun[key]["stagePInLds"]))
+    except (OSError, ValueError, KeyError):
+        pass
 
 
 def _declare():
@@ -56,6 +75,7 @@ def _declare():
     _sig("p4g_lastError", c_char_p)
     _sig("p4g_deviceAvailable", I)
     _sig("p4g_setKernelTiming", None, I)
+    _sig("p4g_setPruneTuning", None, I, I, I)
     _sig("p4g_lastKernelMicroseconds", D, V)
     _sig("p4g_newData", V, I, I)
     _sig("p4g_freeData", None, V)
@@ -114,6 +134,14 @@ def _declare():
     _sig("p4g_getBigPDeck", I, V, I, V)
     _sig("p4g_getCondLikes", I, V, I, V)
     _sig("p4g_newBatch", V, I, V)
+    _sig("p4g_mlpNew", V, I, I, I, ctypes.c_ulonglong)
+    _sig("p4g_mlpFree", None, V)
+    _sig("p4g_mlpNumParams", I, V)
+    _sig("p4g_mlpGetParams", I, V, V)
+    _sig("p4g_mlpSetParams", I, V, V)
+    _sig("p4g_mlpForward", I, V, V, I, V)
+    _sig("p4g_mlpGradients", I, V, V, V, V, I, V, V)
+    _sig("p4g_mlpTrainStep", I, V, V, V, V, I, ctypes.c_float, V)
     _sig("p4g_freeBatch", None, V)
     _sig("p4g_batchLogLike", I, V, V)
 
@@ -155,6 +183,10 @@ def deviceAvailable():
 
 def setKernelTiming(on):
     lib().p4g_setKernelTiming(1 if on else 0)
+
+
+def setPruneTuning(fp32, opsPerGroup, stagePInLds):
+    lib().p4g_setPruneTuning(1 if fp32 else 0, int(opsPerGroup), 1 if stagePInLds else 0)
 
 
 def lastKernelMicroseconds(cTree):
@@ -441,5 +473,67 @@ class Batch:
         try:
             if self.c and _lib is not None:
                 _lib.p4g_freeBatch(self.c)
+        except Exception:
+            pass
+
+
+class Mlp:
+    """The learned proposal's network on the GPU (p4g_mlp*): nIn -> nHidden
+    -> nHidden -> nOut with ReLU, BF16 weights on the MFMA units, FP32
+    accumulation, master weights and Adam."""
+
+    def __init__(self, nIn, nHidden, nOut, seed=0):
+        self.nIn, self.nHidden, self.nOut = nIn, nHidden, nOut
+        self.c = _h(lib().p4g_mlpNew(nIn, nHidden, nOut, seed), "mlpNew")
+        self.nParams = _lib.p4g_mlpNumParams(self.c)
+
+    @staticmethod
+    def _f32(a, cols=None):
+        a = numpy.ascontiguousarray(a, dtype=numpy.float32)
+        if cols is not None and (a.ndim != 2 or a.shape[1] != cols):
+            raise P4Error("Mlp: expected an array of shape [n][%d], got %s" % (cols, a.shape))
+        return a
+
+    def getParams(self):
+        out = numpy.empty(self.nParams, numpy.float32)
+        _ok(_lib.p4g_mlpGetParams(self.c, out.ctypes.data), "mlpGetParams")
+        return out
+
+    def setParams(self, theta):
+        theta = numpy.ascontiguousarray(theta, dtype=numpy.float32)
+        if theta.size != self.nParams:
+            raise P4Error("Mlp.setParams: %d values, expected %d" % (theta.size, self.nParams))
+        _ok(_lib.p4g_mlpSetParams(self.c, theta.ctypes.data), "mlpSetParams")
+
+    def forward(self, x):
+        x = self._f32(x, self.nIn)
+        out = numpy.empty((x.shape[0], self.nOut), numpy.float32)
+        _ok(_lib.p4g_mlpForward(self.c, x.ctypes.data, x.shape[0], out.ctypes.data), "mlpForward")
+        return out
+
+    def _batch(self, x, y, mask):
+        D = self.nOut // 2
+        return self._f32(x, self.nIn), self._f32(y, D), self._f32(mask, D)
+
+    def gradients(self, x, y, mask):
+        """(loss summed over rows, gradient of loss / n)."""
+        x, y, mask = self._batch(x, y, mask)
+        g = numpy.empty(self.nParams, numpy.float32)
+        loss = numpy.zeros(1, numpy.float32)
+        _ok(_lib.p4g_mlpGradients(self.c, x.ctypes.data, y.ctypes.data, mask.ctypes.data, x.shape[0],
+                                  g.ctypes.data, loss.ctypes.data), "mlpGradients")
+        return float(loss[0]), g
+
+    def trainStep(self, x, y, mask, lr):
+        x, y, mask = self._batch(x, y, mask)
+        loss = numpy.zeros(1, numpy.float32)
+        _ok(_lib.p4g_mlpTrainStep(self.c, x.ctypes.data, y.ctypes.data, mask.ctypes.data, x.shape[0],
+                                  float(lr), loss.ctypes.data), "mlpTrainStep")
+        return float(loss[0])
+
+    def __del__(self):
+        try:
+            if self.c and _lib is not None:
+                _lib.p4g_mlpFree(self.c)
         except Exception:
             pass
