@@ -112,6 +112,13 @@ def main():
     algo64, _ = algorithmic_bytes(tree, nPat, 8)
     achieved = algo32 / (kus32 * 1e-6) / 1e9
 
+    kname = "k_prune<4,float,true>"
+    try:
+        with open(pf.TUNING_PATH) as fh:
+            if int(json.load(fh)["fp32"]["opsPerGroup"]) == 0:
+                kname = "k_pruneWave<float,true>"
+    except (OSError, ValueError, KeyError):
+        kname = "k_pruneWave<float,true>"
     traffic = None
     tf = os.path.join(HERE, "profiles", "traffic_het64_fp32.json")
     if os.path.exists(tf):
@@ -145,7 +152,7 @@ def main():
                    "seed": args.seed, "patterns": nPat},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_prune<4,float,true>", "kernel_us": kus32,
+                     "kernel": kname, "kernel_us": kus32,
                      "algorithmic_bytes_per_launch": algo32, "bytes_per_pattern": perpat32},
         "lnL_fp32": lnl32,
         "lnL_fp64": lnl64,

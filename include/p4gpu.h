@@ -222,9 +222,12 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL);
  * evaluation's pruning launches of tree t (summed over parts), -1 if none. */
 void p4g_setKernelTiming(int on);
 /* Tuning of the 4-state pruning kernel (no counterpart in the reference;
- * set by the autotuner, p4gpu/autotune.py): independent binary ops per
- * group (1, 2, 4 or 8) and whether the program's P rows are staged in LDS,
- * for FP64 (fp32 = 0) or FP32 partials.  Results do not depend on it. */
+ * set by the autotuner, p4gpu/autotune.py), for FP64 (fp32 = 0) or FP32
+ * partials: opsPerGroup 0 selects the wave-per-category kernel (the
+ * default); 1, 2, 4 or 8 the grouped kernel with that many independent
+ * binary ops per group, with the program's P rows staged in LDS if
+ * stagePInLds.  Partials already computed are re-laid out (recomputed) at
+ * the next evaluation when the choice changes the layout. */
 void p4g_setPruneTuning(int fp32, int opsPerGroup, int stagePInLds);
 double p4g_lastKernelMicroseconds(const p4g_tree *t);
 

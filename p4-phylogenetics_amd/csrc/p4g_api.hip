@@ -296,12 +296,16 @@ static bool allocTreePart(Tree *t, int pNum)
     ModelPart *mp = t->model->parts[pNum];
     Part *dp = t->data->parts[pNum];
     if (!uploadPart(dp)) return false;
-    if (tp.alloc && tp.precision == t->precision) return true;
+    const bool wave = pruneUsesWave(mp->dim, mp->nCat, t->precision);
+    if (tp.alloc && tp.precision == t->precision && tp.wave == wave) return true;
     if (tp.alloc) {
-        // precision change: the partial buffers are re-laid out
+        // precision or kernel change: the partial buffers are re-laid out
         if (tp.cl) (void)hipFree(tp.cl);
         tp.cl = nullptr;
         devFree(tp.scl);
+        // the block count depends on the kernel
+        devFree(tp.blockSums);
+        devFree(tp.blockBad);
         tp.alloc = false;
         std::fill(tp.dirty.begin(), tp.dirty.end(), 1);
         tp.everComputed = false;
@@ -316,6 +320,8 @@ static bool allocTreePart(Tree *t, int pNum)
     tp.nPat = dp->nPatterns;
     tp.nPatPad = dp->dev.nPatPad;
     tp.precision = t->precision;
+    tp.wave = wave;
+    tp.sclPerSlot = (size_t)(wave ? K : 1) * tp.nPatPad;
     // slots: every non-leaf node, plus one for a leaf root
     t->slotOf.assign(t->nNodes, -1);
     int ns = 0;
@@ -324,7 +330,8 @@ static bool allocTreePart(Tree *t, int pNum)
     tp.nSlots = ns + 1;
     tp.nEig = mp->nComps * mp->nRMatrices;
     const size_t nPS = (size_t)t->nNodes * K * S * S;
-    tp.maxBlocks = tp.nPatPad / std::min(pruneBlockPatternsFor(S, K, t->precision), pruneBlockPatterns(K));
+    // blocks of the launch: the largest count any pruning kernel may use
+    tp.maxBlocks = tp.nPatPad / std::min({pruneBlockPatternsFor(S, K, t->precision), pruneBlockPatterns(K), 64});
     const size_t elt = t->precision == FP32 ? sizeof(float) : sizeof(double);
     tp.clBytes = (size_t)tp.nPatPad * K * S * elt;
     if (!devAlloc(&tp.eigV, (size_t)tp.nEig * S * S, "hipMalloc eigV") ||
@@ -337,7 +344,7 @@ static bool allocTreePart(Tree *t, int pNum)
         !devAlloc(&tp.patLikes, tp.nPatPad, "hipMalloc patLikes"))
         return false;
     if (!tp.cl && !hipOk(hipMalloc(&tp.cl, tp.clBytes * tp.nSlots), "hipMalloc partials")) return false;
-    if (t->precision == FP32 && !devAlloc(&tp.scl, (size_t)tp.nSlots * tp.nPatPad, "hipMalloc scale"))
+    if (t->precision == FP32 && !devAlloc(&tp.scl, (size_t)tp.nSlots * tp.sclPerSlot, "hipMalloc scale"))
         return false;
     if (!tp.hArena) {
         auto up16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
@@ -347,7 +354,7 @@ static bool allocTreePart(Tree *t, int pNum)
         tp.offPack = tp.offEigList + up16(sizeof(int) * tp.nEig);
         tp.offPi = tp.offPack + up16(tp.packBytes);
         tp.offProg = tp.offPi + up16(sizeof(double) * MAX_DIM);
-        tp.progCapWords = 16 * t->nNodes + 64;
+        tp.progCapWords = 24 * t->nNodes + 64;
         tp.arenaBytes = tp.offProg + sizeof(int) * tp.progCapWords;
         if (!hostAlloc(&tp.hArena, tp.arenaBytes, "hipHostMalloc staging arena") ||
             !devAlloc(&tp.dArena, tp.arenaBytes, "hipMalloc staging arena") ||
@@ -610,11 +617,11 @@ static bool bigPDecksPart(Tree *t, Node *nd, int pNum)
 
 // Build the node program for part pNum: all internal nodes in postorder
 // (full) or only the marked ones, always ending with the root.
-static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsOut)
+// The nodes to (re)compute, in postorder, with their children: every
+// non-leaf node when `full`, else the dirty ones; the root last.
+static bool collectOps(Tree *t, int pNum, bool full, std::vector<OpRec> &ops, std::vector<KidRec> &kids)
 {
     TreePart &tp = t->tp[pNum];
-    std::vector<OpRec> ops;
-    std::vector<KidRec> kids;
     std::vector<char> have(t->nNodes, 0);
     for (int n = 0; n < t->nNodes; n++) have[n] = (!t->nodes[n]->isLeaf && tp.everComputed && !tp.dirty[n]);
     for (int j = 0; j < t->nNodes; j++) {
@@ -679,6 +686,22 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
         setError("postOrder does not end at the root");
         return false;
     }
+    return true;
+}
+
+static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids);
+
+static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsOut)
+{
+    TreePart &tp = t->tp[pNum];
+    std::vector<OpRec> ops;
+    std::vector<KidRec> kids;
+    if (!collectOps(t, pNum, full, ops, kids)) return false;
+    if (tp.wave) {
+        *nOpsOut = (int)ops.size();
+        *nKidsOut = (int)kids.size();
+        return buildWaveProgram(t, pNum, ops, kids);
+    }
     // Group ops by height above the nodes already computed: ops of equal
     // height are independent, so binary ones are packed G to a group.
     const int G = pruneGroupWidthFor(t->model->parts[pNum]->dim, t->precision);
@@ -736,6 +759,191 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
     return true;
 }
 
+// Program of the wave kernel.  Order: from the root, children with the
+// larger slot need first (Sethi-Ullman), so few results are live at once.
+// A result goes to an LDS slot when its parent is in the program; the slot
+// is free again once the parent has read it.  Clean children are preloaded
+// into the slots the results never need; the rest are read from HBM.
+static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids)
+{
+    TreePart &tp = t->tp[pNum];
+    const Data *data = t->data;
+    const int nOps = (int)ops.size();
+    std::vector<int> opOfSlot(tp.nSlots, -1), parentOp(nOps, -1), need(nOps, 1);
+    for (int i = 0; i < nOps; i++)
+        if (!(ops[i].flags & OP_ROOT)) opOfSlot[ops[i].dst] = i;
+    std::vector<std::vector<int>> progKids(nOps);
+    for (int i = 0; i < nOps; i++)
+        for (int c = 0; c < ops[i].nKids; c++) {
+            const KidRec &k = kids[ops[i].kidStart + c];
+            if (k.kind == 1 && opOfSlot[k.idx] >= 0 && opOfSlot[k.idx] < i) {
+                progKids[i].push_back(opOfSlot[k.idx]);
+                parentOp[opOfSlot[k.idx]] = i;
+            }
+        }
+    for (int i = 0; i < nOps; i++) {  // postorder: children first
+        auto &pk = progKids[i];
+        std::stable_sort(pk.begin(), pk.end(), [&](int x, int y) { return need[x] > need[y]; });
+        int nd = 1;
+        for (size_t q = 0; q < pk.size(); q++) nd = std::max(nd, need[pk[q]] + (int)q);
+        need[i] = nd;
+    }
+    // depth-first emission from the root (iterative)
+    std::vector<int> order;
+    order.reserve(nOps);
+    {
+        std::vector<std::pair<int, size_t>> st;
+        st.push_back({nOps - 1, 0});
+        while (!st.empty()) {
+            auto &top = st.back();
+            if (top.second < progKids[top.first].size()) {
+                const int c = progKids[top.first][top.second++];
+                st.push_back({c, 0});
+            } else {
+                order.push_back(top.first);
+                st.pop_back();
+            }
+        }
+    }
+    // ops whose parent is not in the program (possible when a node is
+    // marked without its ancestors) run first, in postorder
+    if ((int)order.size() != nOps) {
+        std::vector<char> seen(nOps, 0);
+        for (int i : order) seen[i] = 1;
+        std::vector<int> rest;
+        for (int i = 0; i < nOps; i++)
+            if (!seen[i]) rest.push_back(i);
+        // a stray op may itself have program children: keep postorder
+        order.insert(order.begin(), rest.begin(), rest.end());
+        std::vector<int> pos(nOps);
+        for (int q = 0; q < nOps; q++) pos[order[q]] = q;
+        for (int i = 0; i < nOps; i++)
+            for (int c : progKids[i])
+                if (pos[c] > pos[i]) {
+                    setError("internal: wave program order");
+                    return false;
+                }
+    }
+    const int tipLds = data->nTax <= WAVE_TIP_LDS_MAX_TAXA ? 1 : 0;
+    const int maxSlots = pruneWaveMaxSlots(tp.K, tipLds, data->nTax, t->precision);
+    std::vector<int> pos(nOps);
+    for (int q = 0; q < nOps; q++) pos[order[q]] = q;
+    // pass 1: results only, to find their peak slot use
+    std::vector<int> ldsOf(nOps, -1);
+    auto simulate = [&](int nSlotsAvail, std::vector<int> &ldsOut, int &peak) {
+        std::vector<int> freeList;
+        for (int sl = nSlotsAvail - 1; sl >= 0; sl--) freeList.push_back(sl);
+        std::fill(ldsOut.begin(), ldsOut.end(), -1);
+        int used = 0;
+        peak = 0;
+        for (int q = 0; q < nOps; q++) {
+            const int i = order[q];
+            for (int c : progKids[i])
+                if (ldsOut[c] >= 0) {
+                    freeList.push_back(ldsOut[c]);
+                    used--;
+                }
+            if (parentOp[i] >= 0 && !freeList.empty()) {
+                ldsOut[i] = freeList.back();
+                freeList.pop_back();
+                used++;
+                peak = std::max(peak, used);
+            }
+        }
+    };
+    int peakRes = 0;
+    simulate(maxSlots, ldsOf, peakRes);
+    // clean internal children, by consumer position
+    struct Pre {
+        int q, op, c, cl;
+    };
+    std::vector<Pre> cleans;
+    for (int q = 0; q < nOps; q++) {
+        const int i = order[q];
+        for (int c = 0; c < ops[i].nKids; c++) {
+            const KidRec &k = kids[ops[i].kidStart + c];
+            if (k.kind == 1 && !(opOfSlot[k.idx] >= 0 && opOfSlot[k.idx] < i)) cleans.push_back({q, i, c, k.idx});
+        }
+    }
+    // every preload is loaded at the start, so each takes a slot of its own
+    // above the results' [0, peakRes); the first consumers get them
+    const int nPreSlots = std::max(0, maxSlots - peakRes);
+    std::vector<int> preSlot(cleans.size(), -1);
+    int nPre = 0;
+    for (size_t j = 0; j < cleans.size() && nPre < nPreSlots; j++) preSlot[j] = peakRes + nPre++;
+    const int nLds = peakRes + nPre;
+    // preloaded child lookup by (op, child index)
+    std::vector<std::vector<int>> preOf(nOps);
+    for (int i = 0; i < nOps; i++) preOf[i].assign(ops[i].nKids, -1);
+    for (size_t j = 0; j < cleans.size(); j++) preOf[cleans[j].op][cleans[j].c] = preSlot[j];
+    // emit
+    int nSide = 0;
+    for (int i = 0; i < nOps; i++) nSide += std::max(0, ops[i].nKids - 3);
+    const size_t words = (size_t)16 * nOps + 4 * (size_t)nSide + 2 * (size_t)nPre;
+    if ((int)words > tp.progCapWords) {
+        setError("internal: node program larger than its staging area");
+        return false;
+    }
+    WaveOp *wo = reinterpret_cast<WaveOp *>(tp.hProg);
+    WaveKid *wk = reinterpret_cast<WaveKid *>(tp.hProg + 16 * nOps);
+    int *wp = tp.hProg + 16 * nOps + 4 * nSide;
+    int side = 0;
+    for (int q = 0; q < nOps; q++) {
+        const int i = order[q];
+        WaveOp w{};
+        w.dst = (ops[i].flags & OP_ROOT) ? -1 : ops[i].dst;
+        w.ldsDst = ldsOf[i];
+        w.flags = ops[i].flags;
+        w.nKids = ops[i].nKids;
+        w.kidStart = side;
+        for (int c = 0; c < ops[i].nKids; c++) {
+            const KidRec &k = kids[ops[i].kidStart + c];
+            int kind, idx;
+            if (k.kind == 0) {
+                kind = WK_TIP;
+                idx = k.idx;
+            } else {
+                const int j = opOfSlot[k.idx];
+                if (j >= 0 && j < i && pos[j] < q) {
+                    kind = ldsOf[j] >= 0 ? WK_LDS : WK_GLOBAL;
+                    idx = ldsOf[j] >= 0 ? ldsOf[j] : k.idx;
+                } else if (preOf[i][c] >= 0) {
+                    kind = WK_LDS;
+                    idx = preOf[i][c];
+                } else {
+                    kind = WK_GLOBAL;
+                    idx = k.idx;
+                }
+            }
+            if (c < 3) {
+                w.kind[c] = kind;
+                w.idx[c] = idx;
+                w.pnode[c] = k.pnode;
+            } else {
+                wk[side++] = WaveKid{kind, idx, k.pnode, 0};
+            }
+        }
+        wo[q] = w;
+    }
+    int np = 0;
+    for (size_t j = 0; j < cleans.size(); j++)
+        if (preSlot[j] >= 0) {
+            wp[2 * np] = cleans[j].cl;
+            wp[2 * np + 1] = preSlot[j];
+            np++;
+        }
+    tp.nWOps = nOps;
+    tp.nWKids = nSide;
+    tp.nPre = nPre;
+    tp.nLds = nLds;
+    tp.tipLds = tipLds;
+    tp.waveLds = pruneWaveLdsBytes(tp.K, nLds, tipLds, data->nTax, t->precision);
+    tp.progWords = (int)words;
+    tp.nGroups = 0;
+    tp.nKids = (int)kids.size();
+    return true;
+}
+
 static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLikes)
 {
     TreePart &tp = t->tp[pNum];
@@ -770,6 +978,16 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     a.counter = tp.counter;
     a.result = tp.result;
     a.patLikes = siteLikes ? tp.patLikes : nullptr;
+    if (tp.wave) {
+        a.wops = reinterpret_cast<const WaveOp *>(prog);
+        a.wkids = reinterpret_cast<const WaveKid *>(prog + 16 * tp.nWOps);
+        a.wpre = prog + 16 * tp.nWOps + 4 * tp.nWKids;
+        a.nWOps = tp.nWOps;
+        a.nPre = tp.nPre;
+        a.nLds = tp.nLds;
+        a.tipLds = tp.tipLds;
+        a.nTax = dp->nTax;
+    }
 }
 
 // Stage the program and root frequencies of one part (async).
@@ -813,10 +1031,11 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
     PruneArgs a{};
     fillPruneArgs(t, pNum, nOps, a, siteLikes);
     hipStream_t s = stream();
-    const int nBlocks = tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision);
+    const int nBlocks = tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision);
     if (g_timing) (void)hipEventRecord(tp.ev0, s);
-    const hipError_t le = pruneUsesMfma(a.dim, a.K, t->precision) ? launchPruneAa(a, nBlocks, s)
-                                                                   : launchPrune(a, t->precision, nBlocks, s);
+    const hipError_t le = tp.wave ? launchPruneWave(a, t->precision, nBlocks, tp.waveLds, s)
+                          : pruneUsesMfma(a.dim, a.K, t->precision) ? launchPruneAa(a, nBlocks, s)
+                                                                    : launchPrune(a, t->precision, nBlocks, s);
     if (!hipOk(le, "launch pruning kernel")) return false;
     if (g_timing) (void)hipEventRecord(tp.ev1, s);
     if (!hipOk(hipMemcpyAsync(tp.hResult, tp.result, 2 * sizeof(double), hipMemcpyDeviceToHost, s),
@@ -1545,14 +1764,14 @@ int p4g_copyCondLikes(p4g_tree *a, p4g_tree *b, int doAll)
     hipStream_t s = stream();
     for (int p = 0; p < a->nParts; p++) {
         TreePart &A = a->tp[p], &B = b->tp[p];
-        if (A.nSlots != B.nSlots || A.clBytes != B.clBytes) {
+        if (A.nSlots != B.nSlots || A.clBytes != B.clBytes || A.wave != B.wave || A.sclPerSlot != B.sclPerSlot) {
             setError("copyCondLikes: partial buffers differ");
             return 1;
         }
         if (doAll) {
             if (!hipOk(hipMemcpyAsync(B.cl, A.cl, A.clBytes * A.nSlots, hipMemcpyDeviceToDevice, s), "copy partials"))
                 return 1;
-            if (A.scl && !hipOk(hipMemcpyAsync(B.scl, A.scl, sizeof(int) * (size_t)A.nSlots * A.nPatPad,
+            if (A.scl && !hipOk(hipMemcpyAsync(B.scl, A.scl, sizeof(int) * (size_t)A.nSlots * A.sclPerSlot,
                                                hipMemcpyDeviceToDevice, s),
                                 "copy scale exponents"))
                 return 1;
@@ -1567,8 +1786,8 @@ int p4g_copyCondLikes(p4g_tree *a, p4g_tree *b, int doAll)
                                           hipMemcpyDeviceToDevice, s),
                            "copy partials"))
                     return 1;
-                if (A.scl && !hipOk(hipMemcpyAsync(B.scl + (size_t)sl * A.nPatPad, A.scl + (size_t)sl * A.nPatPad,
-                                                   sizeof(int) * A.nPatPad, hipMemcpyDeviceToDevice, s),
+                if (A.scl && !hipOk(hipMemcpyAsync(B.scl + (size_t)sl * A.sclPerSlot, A.scl + (size_t)sl * A.sclPerSlot,
+                                                   sizeof(int) * A.sclPerSlot, hipMemcpyDeviceToDevice, s),
                                     "copy scale exponents"))
                     return 1;
                 B.dirty[i] = A.dirty[i];
@@ -1718,9 +1937,9 @@ int p4g_getCondLikes(const p4g_node *n, int pNum, double *out)
     std::vector<double> v(per);
     if (t->precision == FP32) {
         std::vector<float> f(per);
-        e.resize(tp.nPatPad);
+        e.resize(tp.sclPerSlot);
         if (!hipOk(hipMemcpy(f.data(), (const char *)tp.cl + tp.clBytes * sl, per * 4, hipMemcpyDeviceToHost), "copy") ||
-            !hipOk(hipMemcpy(e.data(), tp.scl + (size_t)sl * tp.nPatPad, e.size() * 4, hipMemcpyDeviceToHost), "copy"))
+            !hipOk(hipMemcpy(e.data(), tp.scl + (size_t)sl * tp.sclPerSlot, e.size() * 4, hipMemcpyDeviceToHost), "copy"))
             return 1;
         for (size_t i = 0; i < per; i++) v[i] = f[i];
     } else if (!hipOk(hipMemcpy(v.data(), (const char *)tp.cl + tp.clBytes * sl, per * 8, hipMemcpyDeviceToHost), "copy"))
@@ -1729,10 +1948,12 @@ int p4g_getCondLikes(const p4g_node *n, int pNum, double *out)
     for (int k = 0; k < K; k++)
         for (int s = 0; s < S; s++)
             for (int p = 0; p < nP; p++) {
-                // tile-blocked layout of the MFMA kernel: [tile][k][s/4][s%4][p%16]
-                double x = tiled ? v[(size_t)(p / 16) * K * S * 16 + ((size_t)(k * 5 + s / 4) * 4 + s % 4) * 16 + p % 16]
-                                 : v[((size_t)p * K + k) * S + s];
-                if (!e.empty()) x = std::ldexp(x, -e[p]);
+                // wave layout [k][p][s]; tile-blocked layout of the MFMA
+                // kernel [tile][k][s/4][s%4][p%16]; else [p][k][s]
+                double x = tp.wave ? v[((size_t)k * tp.nPatPad + p) * S + s]
+                           : tiled ? v[(size_t)(p / 16) * K * S * 16 + ((size_t)(k * 5 + s / 4) * 4 + s % 4) * 16 + p % 16]
+                                   : v[((size_t)p * K + k) * S + s];
+                if (!e.empty()) x = std::ldexp(x, -e[tp.wave ? (size_t)k * tp.nPatPad + p : (size_t)p]);
                 out[((size_t)k * S + s) * nP + p] = x;
             }
     return 0;
@@ -1799,7 +2020,8 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     int nBlocksTot = 0;
     auto itemBlocks = [](const Item &it) {
         const ModelPart *mp = it.t->model->parts[it.p];
-        return it.t->tp[it.p].nPatPad / pruneBlockPatternsFor(mp->dim, mp->nCat, it.t->precision);
+        const TreePart &tp = it.t->tp[it.p];
+        return tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(mp->dim, mp->nCat, it.t->precision);
     };
     for (auto &it : items) nBlocksTot += itemBlocks(it);
     if (nItems > b->capItems) {
@@ -1827,7 +2049,9 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         const ModelPart *mp = items[i].t->model->parts[items[i].p];
         const int prec = items[i].t->precision;
         const int kc = pruneUsesMfma(mp->dim, mp->nCat, prec) ? (mp->nCat <= 1 ? 1 : mp->nCat <= 2 ? 2 : mp->nCat <= 4 ? 4 : 8) : 0;
-        return (mp->dim * 2 + prec) * 16 + kc;
+        // the wave kernel's workgroup size is 64 * K: one launch per K
+        const int wk = items[i].t->tp[items[i].p].wave ? 16 + mp->nCat : 0;
+        return ((mp->dim * 2 + prec) * 16 + kc) * 32 + wk;
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
     for (int j = 0; j < nItems; j++) {
@@ -1842,7 +2066,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     std::vector<std::pair<int, int>> launches;  // (first block, count) per group
     std::vector<int> groupDim, groupPrec;
     std::vector<size_t> groupLds;
-    std::vector<int> groupK;
+    std::vector<int> groupK, groupWave;
     while (j < nItems) {
         const int k0 = key(order[j]);
         const int first = blk;
@@ -1861,16 +2085,23 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         groupDim.push_back(items[order[j]].t->model->parts[items[order[j]].p]->dim);
         groupPrec.push_back(items[order[j]].t->precision);
         groupK.push_back(items[order[j]].t->model->parts[items[order[j]].p]->nCat);
-        // P in LDS for the whole launch only if every item's P rows fit
+        groupWave.push_back(items[order[j]].t->tp[items[order[j]].p].wave ? 1 : 0);
         size_t lds = 0;
-        bool fits = true;
-        for (int q = j; q < jj; q++) {
-            const PruneArgs &pa = b->hArgs[q];
-            const size_t l = pruneLdsBytes(pa.nKids, pa.K, pa.dim, groupPrec.back());
-            if (!l) fits = false;
-            lds = std::max(lds, l);
+        if (groupWave.back()) {
+            // the largest slot file of the launch's items
+            for (int q = j; q < jj; q++) lds = std::max(lds, items[order[q]].t->tp[items[order[q]].p].waveLds);
+            groupLds.push_back(lds);
+        } else {
+            // P in LDS for the whole launch only if every item's P rows fit
+            bool fits = true;
+            for (int q = j; q < jj; q++) {
+                const PruneArgs &pa = b->hArgs[q];
+                const size_t l = pruneLdsBytes(pa.nKids, pa.K, pa.dim, groupPrec.back());
+                if (!l) fits = false;
+                lds = std::max(lds, l);
+            }
+            groupLds.push_back(fits ? lds : 0);
         }
-        groupLds.push_back(fits ? lds : 0);
         j = jj;
     }
     if (!hipOk(hipMemcpyAsync(b->dBlockItem, b->hBlockItem, sizeof(int) * 2 * blk, hipMemcpyHostToDevice, s),
@@ -1878,7 +2109,9 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         return 1;
     for (size_t g = 0; g < launches.size(); g++) {
         const int *bi = b->dBlockItem + 2 * launches[g].first;
-        const hipError_t le = pruneUsesMfma(groupDim[g], groupK[g], groupPrec[g])
+        const hipError_t le = groupWave[g] ? launchPruneWaveBatch(b->dArgs, bi, launches[g].second, groupK[g],
+                                                                 groupPrec[g], groupLds[g], s)
+                              : pruneUsesMfma(groupDim[g], groupK[g], groupPrec[g])
                                   ? launchPruneAaBatch(b->dArgs, bi, launches[g].second, groupK[g], s)
                                   : launchPruneBatch(b->dArgs, bi, launches[g].second, groupDim[g], groupPrec[g],
                                                      groupLds[g], s);

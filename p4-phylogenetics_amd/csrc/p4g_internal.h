@@ -136,6 +136,29 @@ struct KidRec {
     int pad;
 };
 constexpr int OP_ROOT = 1;
+
+// Program of the wave-per-category DNA kernel (k_pruneWave): one 16-int
+// record per node, in execution order, children inline (up to 3; more in
+// a side array of 4-int records).  A child is a tip row, a slot of the
+// wave's LDS slot file (a partial computed earlier in the same launch or
+// preloaded at its start), or a partials slot in global memory.
+enum { WK_TIP = 0, WK_LDS = 1, WK_GLOBAL = 2 };
+struct WaveOp {
+    int dst;       // partials slot written (the root: -1)
+    int ldsDst;    // LDS slot that also receives the result, or -1
+    int flags;     // OP_ROOT
+    int nKids;
+    int kidStart;  // index of the 4th child in the side array
+    int kind[3], idx[3], pnode[3];
+    int pad[2];
+};
+static_assert(sizeof(WaveOp) == 64, "WaveOp is one 64-byte scalar load");
+struct WaveKid {
+    int kind, idx, pnode, pad;
+};
+constexpr int WAVE_MAX_K = 8;             // categories (waves per workgroup)
+constexpr int WAVE_MAX_LDS_SLOTS = 16;
+constexpr int WAVE_TIP_LDS_MAX_TAXA = 256;
 // Group of ops processed together: kind 0 = up to G independent binary,
 // non-root ops; kind 1 = one op with any number of children (or the root).
 struct GroupRec {
@@ -160,7 +183,8 @@ struct TreePart {
     double *PT64 = nullptr;
     float *PT32 = nullptr;
     // Partials: [nSlots][nPatPad][K][dim] of float or double, scale
-    // exponents [nSlots][nPatPad] (FP32 mode).
+    // exponents [nSlots][nPatPad] (FP32 mode); the wave kernel's layout is
+    // [nSlots][K][nPatPad][dim] with exponents [nSlots][K][nPatPad].
     void *cl = nullptr;
     int *scl = nullptr;
     size_t clBytes = 0;
@@ -173,6 +197,13 @@ struct TreePart {
     int maxBlocks = 0;
     // Staged program (device pointers into the arena).
     int nGroups = 0, nKids = 0, progWords = 0;
+    // Wave-kernel program: records, side kids, preloads, LDS slots per
+    // wave, tips in LDS; per-slot exponent count (K*nPatPad for the wave
+    // layout, else nPatPad).
+    bool wave = false;
+    int nWOps = 0, nWKids = 0, nPre = 0, nLds = 0, tipLds = 0;
+    size_t waveLds = 0;
+    size_t sclPerSlot = 0;
     // Host-side flags.
     std::vector<char> dirty;       // per node
     std::vector<char> pValid;      // per node
@@ -260,6 +291,13 @@ struct PruneArgs {
     unsigned *counter;
     double *result;
     double *patLikes;        // may be null
+    // wave-per-category DNA kernel
+    const WaveOp *wops;
+    const WaveKid *wkids;
+    const int *wpre;         // [nPre][2]: partials slot, LDS slot
+    int nWOps, nPre, nLds;   // nLds: LDS slots per wave
+    int tipLds;              // 1: the tile's tip codes are staged in LDS
+    int nTax;
 };
 
 // Kernel launchers (p4g_kernels.hip).
@@ -277,6 +315,15 @@ bool pruneUsesMfma(int dim, int K, int precision);
 int pruneBlockPatternsFor(int dim, int K, int precision);
 hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s);
 hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, hipStream_t s);
+// 4-state parts with at most WAVE_MAX_K categories use the wave-per-category
+// kernel: partials [slot][cat][pattern][state], FP32 exponents
+// [slot][cat][pattern]; 64 patterns per workgroup of 64*K threads.
+bool pruneUsesWave(int dim, int K, int precision);
+size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int precision);
+int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision);
+hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s);
+hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, int precision, size_t lds,
+                                hipStream_t s);
 
 // Host numerics (p4g_host.hip).
 int discreteGamma(double *freqK, double *rK, double alfa, double beta, int K, int median);

@@ -54,44 +54,43 @@ __global__ __launch_bounds__(64) void k_eig(EigArgs a)
     const double *R = pi + S;
     const int lane = threadIdx.x;
 
+    // Q = R . diag(pi), diagonal = -row sum, scaled to mean rate 1
+    // (Pf/util.c:44-115), built in parallel in U's storage, then
+    // symmetrised into A = diag(pi)^(1/2) Q diag(pi)^(-1/2).
+    double *Q = U;
+    double *isq = cs;          // [n], free until the sweeps start
+    double *rowS = A;          // row sums, in A's first row before A is built
+    for (int i = lane; i < n; i += 64) {
+        const double v = (i < S) ? sqrt(pi[i]) : 1.0;
+        sq[i] = v;
+        isq[i] = 1.0 / v;
+    }
     for (int i = lane; i < n * n; i += 64) {
-        A[i] = 0.0;
-        U[i] = ((i / n) == (i % n)) ? 1.0 : 0.0;
+        const int r = i / n, c = i % n;
+        Q[i] = (r < S && c < S && r != c) ? R[r * S + c] * pi[c] : 0.0;
     }
-    for (int i = lane; i < n; i += 64) sq[i] = (i < S) ? sqrt(pi[i]) : 1.0;
     __syncthreads();
-    if (lane == 0) {
-        // Q = R . diag(pi), diagonal = -row sum, scaled to mean rate 1,
-        // in the operation order of Pf/util.c:44-115.  Q is built in U's
-        // storage and then symmetrised into A.
-        double *Q = U;
+    for (int r = lane; r < S; r += 64) {
+        double sum = 0.0;
         for (int c = 0; c < S; c++)
-            for (int r = 0; r < S; r++) Q[r * n + c] = R[r * S + c] * pi[c];
-        for (int r = 0; r < S; r++) {
-            double sum = 0.0;
-            for (int c = 0; c < S; c++)
-                if (r != c) sum = sum + Q[r * n + c];
-            Q[r * n + r] = -sum;
-        }
-        double sumODE = 0.0;
-        for (int r = 0; r < S; r++)
-            for (int c = 0; c < S; c++)
-                if (r != c) sumODE = sumODE + pi[r] * Q[r * n + c];
-        sumODE = 1.0 / sumODE;
-        for (int r = 0; r < S; r++)
-            for (int c = 0; c < S; c++) Q[r * n + c] = Q[r * n + c] * sumODE;
-        for (int i = 0; i < S; i++)
-            for (int j = 0; j < S; j++) {
-                if (i == j) {
-                    A[i * n + i] = Q[i * n + i];
-                } else {
-                    double aij = sq[i] * Q[i * n + j] / sq[j];
-                    double aji = sq[j] * Q[j * n + i] / sq[i];
-                    A[i * n + j] = 0.5 * (aij + aji);
-                }
-            }
-        for (int i = 0; i < n * n; i++) U[i] = ((i / n) == (i % n)) ? 1.0 : 0.0;
+            if (r != c) sum += Q[r * n + c];
+        Q[r * n + r] = -sum;
+        rowS[r] = pi[r] * sum;
     }
+    __syncthreads();
+    double sumODE = 0.0;     // sum_r pi_r sum_{c != r} Q_rc, the same on every lane
+    for (int r = 0; r < S; r++) sumODE += rowS[r];
+    const double inv = 1.0 / sumODE;
+    __syncthreads();
+    for (int i = lane; i < n * n; i += 64) {
+        const int r = i / n, c = i % n;
+        double v = 0.0;
+        if (r < S && c < S)
+            v = (r == c) ? Q[i] * inv : 0.5 * inv * (sq[r] * Q[i] * isq[c] + sq[c] * Q[c * n + r] * isq[r]);
+        A[i] = v;
+    }
+    __syncthreads();
+    for (int i = lane; i < n * n; i += 64) U[i] = ((i / n) == (i % n)) ? 1.0 : 0.0;
     __syncthreads();
 
     for (int sweep = 0; sweep < 30; sweep++) {
@@ -636,10 +635,18 @@ __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneBatch(const PruneArgs *a
 // whether P rows are staged in LDS.  Other state counts use 1 op per group.
 static int g_tuneG[2] = {4, 4};      // [precision]
 static int g_tuneLds[2] = {1, 1};
+static int g_wave[2] = {1, 1};       // 1: 4-state parts use k_pruneWave
 
+// G = 0 selects the wave-per-category kernel for 4-state parts; G = 1, 2,
+// 4 or 8 the grouped kernel with that many ops per group.
 void setPruneTuning(int precision, int G, int lds)
 {
     if (precision < 0 || precision > 1) return;
+    if (G == 0) {
+        g_wave[precision] = 1;
+        return;
+    }
+    g_wave[precision] = 0;
     if (G == 1 || G == 2 || G == 4 || G == 8) g_tuneG[precision] = G;
     g_tuneLds[precision] = lds ? 1 : 0;
 }
@@ -946,6 +953,325 @@ hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, in
     case 4: hipLaunchKernelGGL(k_pruneAaBatch<4>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi); break;
     default: hipLaunchKernelGGL(k_pruneAaBatch<8>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi); break;
     }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------ wave-per-category DNA
+// One wavefront per (64-pattern tile, gamma category); a workgroup holds
+// the K category waves of one tile and they share nothing but the staged
+// tip codes and the root combine.  Partials [slot][cat][pattern][4].
+// A child computed earlier in the same launch, or a clean child listed
+// for preload, is read from the wave's LDS slot file, so the node program
+// never waits on HBM between nodes; every computed partial is still
+// written to HBM, where later incremental evaluations read it.  P^T and
+// the program records are wave-uniform and come through scalar loads.
+// FP32 exponents are per (node, pattern, category), so no lane waits on
+// another until the root, where one barrier combines the categories.
+typedef const __attribute__((address_space(4))) int cint_t;
+template <typename T> struct ConstT;
+template <> struct ConstT<float> { typedef const __attribute__((address_space(4))) float type; };
+template <> struct ConstT<double> { typedef const __attribute__((address_space(4))) double type; };
+template <typename T> struct Vec4T;
+template <> struct Vec4T<float> { typedef float4 type; };
+template <> struct Vec4T<double> { typedef double4 type; };
+
+constexpr int WAVE_LDS_BUDGET = 48 * 1024;   // dynamic LDS per workgroup
+
+// leafVec for 4 states with P^T in registers: the lane's code selects
+// columns by masks, never by an index into the register array.  Same
+// values and summation order as leafVec.
+template <typename T>
+__device__ __forceinline__ void leafVec4(const T *Pl, unsigned code, const uint64_t *amb, T *v)
+{
+    unsigned m;
+    if (code < 4u)
+        m = 1u << code;
+    else if (code == TIP_ALL)
+        m = 0u;
+    else
+        m = (unsigned)amb[code - 4] & 15u;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        T x = T(0);
+#pragma unroll
+        for (int j = 0; j < 4; j++) x += ((m >> j) & 1u) ? Pl[j * 4 + s] : T(0);
+        v[s] = (code == TIP_ALL) ? T(1) : x;
+    }
+}
+
+// The last workgroup of the item sums the per-workgroup partials (wave 0
+// of each workgroup reports) in a fixed order with Neumaier compensation.
+__device__ __forceinline__ void finishSiteSumWave(const PruneArgs &a, double contrib, int bad, int blk, int nBlk)
+{
+    const int lane = threadIdx.x & 63;
+    for (int off = 32; off >= 1; off >>= 1) {
+        contrib += __shfl_xor(contrib, off);
+        bad |= __shfl_xor(bad, off);
+    }
+    unsigned prev = 0;
+    if (lane == 0) {
+        a.blockSums[blk] = contrib;
+        a.blockBad[blk] = bad;
+        __threadfence();
+        prev = __hip_atomic_fetch_add(a.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    prev = __shfl(prev, 0);
+    if (prev != (unsigned)(nBlk - 1)) return;
+    __threadfence();
+    double s = 0.0, comp = 0.0;
+    int b = 0;
+    for (int i = lane; i < nBlk; i += 64) {
+        const double x = __hip_atomic_load(a.blockSums + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b |= __hip_atomic_load(a.blockBad + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double t = s + x;  // Neumaier
+        if (fabs(s) >= fabs(x))
+            comp += (s - t) + x;
+        else
+            comp += (x - t) + s;
+        s = t;
+    }
+    double v = s + comp;
+    for (int off = 32; off >= 1; off >>= 1) {
+        v += __shfl_xor(v, off);
+        b |= __shfl_xor(b, off);
+    }
+    if (lane == 0) {
+        a.result[0] = b ? -1.0e99 : v;
+        a.result[1] = (double)b;
+        *a.counter = 0u;
+    }
+}
+
+template <typename T, bool SCALE>
+__device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int nTiles)
+{
+    typedef typename Vec4T<T>::type V4;
+    typedef typename ConstT<T>::type cT;
+    extern __shared__ __align__(16) unsigned char smemW[];
+    __shared__ double rootL[WAVE_MAX_K][64];
+    __shared__ int rootE[WAVE_MAX_K][64];
+    const int K = a.K;
+    const int cat = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int p = tile * 64 + lane;
+    const size_t nPP = (size_t)a.nPatPad;
+    const int nL = a.nLds;
+    const int tipBytes = a.tipLds ? ((a.nTax * 64 + 15) & ~15) : 0;
+    const unsigned char *sTip = smemW;
+    V4 *slot = reinterpret_cast<V4 *>(smemW + tipBytes) + (size_t)cat * nL * 64 + lane;
+    int *sexp = reinterpret_cast<int *>(smemW + tipBytes + sizeof(V4) * (size_t)K * nL * 64) +
+                (size_t)cat * nL * 64 + lane;
+    if (a.tipLds) {
+        // the tile's tip codes, [taxon][64 patterns], 4 bytes per thread
+        const int words = a.nTax * 16;
+        for (int w = threadIdx.x; w < words; w += blockDim.x) {
+            const int r = w >> 4, q = w & 15;
+            reinterpret_cast<uint32_t *>(smemW)[w] =
+                *reinterpret_cast<const uint32_t *>(a.tips + (size_t)r * nPP + (size_t)tile * 64 + q * 4);
+        }
+        __syncthreads();
+    }
+    const V4 *clr = reinterpret_cast<const V4 *>(a.cl);
+    V4 *clw = reinterpret_cast<V4 *>(a.cl);
+    const size_t catOff = (size_t)cat * nPP + p;   // element of slot s: s*K*nPP + catOff
+    const size_t slotStride = (size_t)K * nPP;
+
+    // clean children read by the program: loads issued in pairs before
+    // their LDS stores
+    cint_t *pre = (cint_t *)(a.wpre);
+    for (int i = 0; i < a.nPre; i += 2) {
+        const bool two = i + 1 < a.nPre;
+        const size_t o0 = (size_t)pre[2 * i] * slotStride + catOff;
+        const size_t o1 = two ? (size_t)pre[2 * i + 2] * slotStride + catOff : o0;
+        const V4 x0 = clr[o0];
+        const V4 x1 = clr[o1];
+        const int e0 = SCALE ? a.scl[o0] : 0;
+        const int e1 = SCALE ? a.scl[o1] : 0;
+        const int l0 = pre[2 * i + 1];
+        slot[l0 * 64] = x0;
+        if (SCALE) sexp[l0 * 64] = e0;
+        if (two) {
+            const int l1 = pre[2 * i + 3];
+            slot[l1 * 64] = x1;
+            if (SCALE) sexp[l1 * 64] = e1;
+        }
+    }
+
+    cint_t *ops = (cint_t *)(a.wops);
+    cint_t *side = (cint_t *)(a.wkids);
+    cT *PT = (cT *)(a.PT);
+    double contrib = 0.0;
+    int bad = 0;
+    for (int i = 0; i < a.nWOps; i++) {
+        cint_t *op = ops + 16 * i;
+        const int nk = op[3];
+        T acc[4];
+        int e = 0;
+        for (int c = 0; c < nk; c++) {
+            int kind, idx, pn;
+            if (c < 3) {
+                kind = op[5 + c];
+                idx = op[8 + c];
+                pn = op[11 + c];
+            } else {
+                cint_t *kr = side + 4 * (op[4] + c - 3);
+                kind = kr[0];
+                idx = kr[1];
+                pn = kr[2];
+            }
+            cT *Pk = PT + ((size_t)pn * K + cat) * 16;
+            T Pl[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) Pl[j] = Pk[j];
+            T v[4];
+            bool skip = false;
+            if (kind == WK_TIP) {
+                const unsigned code = a.tipLds ? sTip[idx * 64 + lane] : a.tips[(size_t)idx * nPP + p];
+                leafVec4<T>(Pl, code, a.ambMask, v);
+                skip = (code == TIP_ALL) && c != 0;
+            } else {
+                V4 x;
+                int ex = 0;
+                if (kind == WK_LDS) {
+                    x = slot[idx * 64];
+                    if (SCALE) ex = sexp[idx * 64];
+                } else {
+                    const size_t o = (size_t)idx * slotStride + catOff;
+                    x = clr[o];
+                    if (SCALE) ex = a.scl[o];
+                }
+                const T xv[4] = {x.x, x.y, x.z, x.w};
+                internalVec<4, T>(Pl, xv, 4, v);
+                e += ex;
+            }
+            if (c == 0) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc[s] = v[s];
+            } else if (!skip) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc[s] *= v[s];
+            }
+        }
+        if (SCALE) {
+            const T m = fmax(fmax(acc[0], acc[1]), fmax(acc[2], acc[3]));
+            if (m < T(SCALE_THRESH) && m > T(0)) {
+                int ex;
+                frexp(m, &ex);
+                const int E = -ex;
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc[s] = ldexp(acc[s], E);
+                e += E;
+            }
+        }
+        if (!(op[2] & OP_ROOT)) {
+            V4 r;
+            r.x = acc[0];
+            r.y = acc[1];
+            r.z = acc[2];
+            r.w = acc[3];
+            const size_t o = (size_t)op[0] * slotStride + catOff;
+            clw[o] = r;
+            if (SCALE) a.scl[o] = e;
+            const int ld = op[1];
+            if (ld >= 0) {
+                slot[ld * 64] = r;
+                if (SCALE) sexp[ld * 64] = e;
+            }
+            continue;
+        }
+        // ---- root: category-summed site likelihood (Pf/p4_tree.c:1029-1378)
+        const double *pi = a.rootPi;
+        double Lk = 0.0;
+        if (a.rootLeafRow >= 0) {
+            const unsigned code = a.tips[(size_t)a.rootLeafRow * nPP + p];
+            uint64_t m;
+            if (code < 4u)
+                m = 1ull << code;
+            else if (code == TIP_ALL)
+                m = ~0ull;
+            else
+                m = a.ambMask[code - 4];
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+                if ((m >> s) & 1ull) Lk += pi[s] * (double)acc[s];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; s++) Lk += pi[s] * (double)acc[s];
+        }
+        rootL[cat][lane] = Lk;
+        rootE[cat][lane] = e;
+        __syncthreads();
+        if (cat == 0) {
+            int mn = rootE[0][lane];
+            for (int k = 1; k < K; k++) mn = min(mn, rootE[k][lane]);
+            double L = 0.0;
+            for (int k = 0; k < K; k++) L += SCALE ? ldexp(rootL[k][lane], mn - rootE[k][lane]) : rootL[k][lane];
+            siteLogLike<SCALE>(a, L, mn, p, K, pi, contrib, bad);
+        }
+    }
+    if (cat == 0) finishSiteSumWave(a, contrib, bad, tile, nTiles);
+}
+
+template <typename T, bool SCALE>
+__global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneWave(PruneArgs a)
+{
+    pruneWaveBody<T, SCALE>(a, blockIdx.x, gridDim.x);
+}
+
+template <typename T, bool SCALE>
+__global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneWaveBatch(const PruneArgs *args, const int *blockItem)
+{
+    const int item = blockItem[2 * blockIdx.x];
+    const int local = blockItem[2 * blockIdx.x + 1];
+    const PruneArgs a = args[item];
+    pruneWaveBody<T, SCALE>(a, local, a.nPatPad / 64);
+}
+
+bool pruneUsesWave(int dim, int K, int precision)
+{
+    return dim == 4 && K >= 1 && K <= WAVE_MAX_K && g_wave[precision & 1];
+}
+
+static size_t waveTipBytes(int tipLds, int nTax) { return tipLds ? (((size_t)nTax * 64 + 15) & ~(size_t)15) : 0; }
+
+static size_t waveSlotBytes(int K, int precision)
+{
+    const size_t elt = precision == FP32 ? sizeof(float) : sizeof(double);
+    return (size_t)K * 64 * (4 * elt + (precision == FP32 ? sizeof(int) : 0));
+}
+
+size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int precision)
+{
+    return waveTipBytes(tipLds, nTax) + (size_t)nLds * waveSlotBytes(K, precision);
+}
+
+int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision)
+{
+    const size_t tb = waveTipBytes(tipLds, nTax);
+    if (tb >= (size_t)WAVE_LDS_BUDGET) return 0;
+    const size_t n = ((size_t)WAVE_LDS_BUDGET - tb) / waveSlotBytes(K, precision);
+    return (int)std::min<size_t>(n, WAVE_MAX_LDS_SLOTS);
+}
+
+hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s)
+{
+    // lds <= WAVE_LDS_BUDGET, within the default per-workgroup limit
+    const dim3 block(64 * a.K);
+    if (precision == FP32)
+        hipLaunchKernelGGL((k_pruneWave<float, true>), dim3(nBlocks), block, lds, s, a);
+    else
+        hipLaunchKernelGGL((k_pruneWave<double, false>), dim3(nBlocks), block, lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, int precision, size_t lds,
+                                hipStream_t s)
+{
+    const dim3 block(64 * K);
+    if (precision == FP32)
+        hipLaunchKernelGGL((k_pruneWaveBatch<float, true>), dim3(nBlocks), block, lds, s, d, bi);
+    else
+        hipLaunchKernelGGL((k_pruneWaveBatch<double, false>), dim3(nBlocks), block, lds, s, d, bi);
     return hipGetLastError();
 }
 

@@ -1,10 +1,11 @@
 """Autotuner for the 4-state pruning kernel.
 
-Times every (ops per group, P staged in LDS) variant of the pruning kernel
-on a workload, for FP32 and FP64 partials, and writes the fastest to
-p4gpu/tuning.json, which the library applies when it is loaded.  Variants
-compute identical results (the arithmetic per node is the same); only the
-schedule of loads changes.
+Times the wave-per-category kernel and every (ops per group, P staged in
+LDS) variant of the grouped kernel on a workload, for FP32 and FP64
+partials, and writes the fastest to p4gpu/tuning.json, which the library
+applies when it is loaded (opsPerGroup 0 = the wave kernel).  The grouped
+variants compute identical results; the wave kernel keeps FP32 scale
+exponents per category, which changes FP32 results only at rounding level.
 
     python -m p4gpu.autotune [--workload het64] [--reps 30]
 """
@@ -37,6 +38,10 @@ def autotune(workload="het64", reps=30, seed=1, write=True, verbose=True):
         tree, _ = synth.make_workload(workload, seed=seed, precision=prec)
         tree.calcLogLike(verbose=0)
         res = {}
+        pf.setPruneTuning(prec == "fp32", 0, 1)
+        res[(0, 1)] = time_variant(tree, reps)
+        if verbose:
+            print("%s wave  %.1f us" % (prec, res[(0, 1)]), flush=True)
         for G in (1, 2, 4, 8):
             for lds in (0, 1):
                 pf.setPruneTuning(prec == "fp32", G, lds)
@@ -46,7 +51,7 @@ def autotune(workload="het64", reps=30, seed=1, write=True, verbose=True):
         (G, lds), best = min(res.items(), key=lambda kv: kv[1])
         out[prec] = {"opsPerGroup": G, "stagePInLds": lds, "kernel_us": best,
                      "workload": workload,
-                     "all": {"G%d_lds%d" % k: v for k, v in res.items()}}
+                     "all": {("wave" if k[0] == 0 else "G%d_lds%d" % k): v for k, v in res.items()}}
         pf.setPruneTuning(prec == "fp32", G, lds)
     if write:
         with open(pf.TUNING_PATH, "w") as fh:

@@ -27,7 +27,7 @@ import numpy
 
 from . import synth
 from .func import effectiveSampleSize
-from .learned import LearnedBrLenCompProposal
+from .learned import LearnedProposal
 from .mcmc import Mcmc, compareSplitFrequencies
 
 
@@ -46,6 +46,7 @@ def _summary(m, burnFrac, wall, extra=None):
     }
     for j in range(4):
         ess['comp%d' % j] = effectiveSampleSize(prams[:, j])
+    ess['gammaShape'] = effectiveSampleSize(prams[:, 4])
     freqs = m.splitFrequencies(burnin=b)
     out = dict(gens=m.timing['gens'], seconds=wall, gensPerSecond=m.timing['gens'] / wall, samples=n - b,
                ess=ess, essPerSecond={k: v / wall for k, v in ess.items()},
@@ -77,7 +78,9 @@ def main(argv=None):
     ap.add_argument('--dataSeed', type=int, default=1)
     ap.add_argument('--net', default='learned_net.npz')
     ap.add_argument('--weight', type=float, default=1.0)
-    ap.add_argument('--blockSize', type=int, default=6)
+    ap.add_argument('--wRidge', type=float, default=0.4)
+    ap.add_argument('--wBlock', type=float, default=0.3)
+    ap.add_argument('--wComp', type=float, default=0.3)
     ap.add_argument('--epochs', type=int, default=30)
     ap.add_argument('--burnin', type=float, default=0.1)
     ap.add_argument('--out', default=None)
@@ -109,7 +112,7 @@ def main(argv=None):
     m = Mcmc(tree, nChains=a.chains, seed=a.seed, sampleInterval=a.interval, verbose=False)
     L = None
     if a.phase in ('train', 'learned'):
-        L = LearnedBrLenCompProposal(tree, seed=a.seed, blockSize=a.blockSize)
+        L = LearnedProposal(tree, seed=a.seed, weights=dict(ridge=a.wRidge, block=a.wBlock, comp=a.wComp))
     if a.phase == 'train':
         L.attach(m)
     if a.phase == 'learned':
@@ -130,7 +133,7 @@ def main(argv=None):
         L.save(a.net)
     if a.phase == 'learned':
         extra = dict(learnedAcceptance=L.stats['accepted'] / max(1, L.stats['proposed']),
-                     learnedOutOfBounds=L.stats['outOfBounds'], learnedProposed=L.stats['proposed'])
+                     learnedStats=dict(L.stats))
     summ = _summary(m, a.burnin, wall, extra)
     s = json.dumps(summ)
     if a.out:
