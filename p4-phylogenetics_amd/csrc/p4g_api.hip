@@ -129,6 +129,8 @@ static void freeTreePart(TreePart &t)
     devFree(t.PT32);
     if (t.cl) (void)hipFree(t.cl);
     t.cl = nullptr;
+    if (t.PP) (void)hipFree(t.PP);
+    t.PP = nullptr;
     devFree(t.scl);
     devFree(t.blockSums);
     devFree(t.blockBad);
@@ -344,6 +346,8 @@ static bool allocTreePart(Tree *t, int pNum)
         !devAlloc(&tp.patLikes, tp.nPatPad, "hipMalloc patLikes"))
         return false;
     if (!tp.cl && !hipOk(hipMalloc(&tp.cl, tp.clBytes * tp.nSlots), "hipMalloc partials")) return false;
+    if (!tp.PP && !hipOk(hipMalloc(&tp.PP, (size_t)K * t->nNodes * 16 * sizeof(double)), "hipMalloc staged P"))
+        return false;
     if (t->precision == FP32 && !devAlloc(&tp.scl, (size_t)tp.nSlots * tp.sclPerSlot, "hipMalloc scale"))
         return false;
     if (!tp.hArena) {
@@ -879,7 +883,9 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     // emit
     int nSide = 0;
     for (int i = 0; i < nOps; i++) nSide += std::max(0, ops[i].nKids - 3);
-    const size_t words = (size_t)16 * nOps + 4 * (size_t)nSide + 2 * (size_t)nPre;
+    int nKidTot = 0;
+    for (int i = 0; i < nOps; i++) nKidTot += ops[i].nKids;
+    const size_t words = (size_t)16 * nOps + 4 * (size_t)nSide + 2 * (size_t)nPre + (size_t)nKidTot;
     if ((int)words > tp.progCapWords) {
         setError("internal: node program larger than its staging area");
         return false;
@@ -887,7 +893,8 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     WaveOp *wo = reinterpret_cast<WaveOp *>(tp.hProg);
     WaveKid *wk = reinterpret_cast<WaveKid *>(tp.hProg + 16 * nOps);
     int *wp = tp.hProg + 16 * nOps + 4 * nSide;
-    int side = 0;
+    int *kidP = wp + 2 * nPre;
+    int side = 0, kb = 0;
     for (int q = 0; q < nOps; q++) {
         const int i = order[q];
         WaveOp w{};
@@ -896,6 +903,7 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
         w.flags = ops[i].flags;
         w.nKids = ops[i].nKids;
         w.kidStart = side;
+        w.kidBase = kb;
         for (int c = 0; c < ops[i].nKids; c++) {
             const KidRec &k = kids[ops[i].kidStart + c];
             int kind, idx;
@@ -915,6 +923,7 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
                     idx = k.idx;
                 }
             }
+            kidP[kb++] = k.pnode;
             if (c < 3) {
                 w.kind[c] = kind;
                 w.idx[c] = idx;
@@ -937,7 +946,10 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     tp.nPre = nPre;
     tp.nLds = nLds;
     tp.tipLds = tipLds;
-    tp.waveLds = pruneWaveLdsBytes(tp.K, nLds, tipLds, data->nTax, t->precision);
+    tp.nKidTot = nKidTot;
+    tp.waveLdsNoP = pruneWaveLdsBytes(tp.K, nLds, tipLds, data->nTax, t->precision);
+    tp.pLds = 0;
+    tp.waveLds = tp.waveLdsNoP;
     tp.progWords = (int)words;
     tp.nGroups = 0;
     tp.nKids = (int)kids.size();
@@ -987,6 +999,9 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
         a.nLds = tp.nLds;
         a.tipLds = tp.tipLds;
         a.nTax = dp->nTax;
+        a.PP = tp.PP;
+        a.nKidTot = tp.nKidTot;
+        a.pLds = tp.pLds;
     }
 }
 
@@ -1032,6 +1047,11 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
     fillPruneArgs(t, pNum, nOps, a, siteLikes);
     hipStream_t s = stream();
     const int nBlocks = tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision);
+    if (tp.wave) {
+        const int *kidP = a.wpre + 2 * tp.nPre;
+        if (!hipOk(launchPgather(a.PT, kidP, tp.nKidTot, tp.K, tp.PP, t->precision, s), "launch P gather"))
+            return false;
+    }
     if (g_timing) (void)hipEventRecord(tp.ev0, s);
     const hipError_t le = tp.wave ? launchPruneWave(a, t->precision, nBlocks, tp.waveLds, s)
                           : pruneUsesMfma(a.dim, a.K, t->precision) ? launchPruneAa(a, nBlocks, s)
@@ -2089,7 +2109,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         size_t lds = 0;
         if (groupWave.back()) {
             // the largest slot file of the launch's items
-            for (int q = j; q < jj; q++) lds = std::max(lds, items[order[q]].t->tp[items[order[q]].p].waveLds);
+            for (int q = j; q < jj; q++) lds = std::max(lds, items[order[q]].t->tp[items[order[q]].p].waveLdsNoP);
             groupLds.push_back(lds);
         } else {
             // P in LDS for the whole launch only if every item's P rows fit
@@ -2107,6 +2127,16 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     if (!hipOk(hipMemcpyAsync(b->dBlockItem, b->hBlockItem, sizeof(int) * 2 * blk, hipMemcpyHostToDevice, s),
                "stage batch map"))
         return 1;
+    // P^T of each wave item's children in program order
+    for (int q = 0; q < nItems; q++) {
+        const Item &it = items[order[q]];
+        const TreePart &tp = it.t->tp[it.p];
+        if (!tp.wave) continue;
+        const PruneArgs &pa = b->hArgs[q];
+        if (!hipOk(launchPgather(pa.PT, pa.wpre + 2 * pa.nPre, tp.nKidTot, tp.K, tp.PP, it.t->precision, s),
+                   "launch P gather"))
+            return 1;
+    }
     for (size_t g = 0; g < launches.size(); g++) {
         const int *bi = b->dBlockItem + 2 * launches[g].first;
         const hipError_t le = groupWave[g] ? launchPruneWaveBatch(b->dArgs, bi, launches[g].second, groupK[g],

@@ -150,7 +150,8 @@ struct WaveOp {
     int nKids;
     int kidStart;  // index of the 4th child in the side array
     int kind[3], idx[3], pnode[3];
-    int pad[2];
+    int pad;
+    int kidBase;   // program-order index of the first child (staged P)
 };
 static_assert(sizeof(WaveOp) == 64, "WaveOp is one 64-byte scalar load");
 struct WaveKid {
@@ -201,8 +202,9 @@ struct TreePart {
     // wave, tips in LDS; per-slot exponent count (K*nPatPad for the wave
     // layout, else nPatPad).
     bool wave = false;
-    int nWOps = 0, nWKids = 0, nPre = 0, nLds = 0, tipLds = 0;
-    size_t waveLds = 0;
+    int nWOps = 0, nWKids = 0, nPre = 0, nLds = 0, tipLds = 0, nKidTot = 0, pLds = 0;
+    void *PP = nullptr;            // [nNodes][K][16] of float or double
+    size_t waveLds = 0, waveLdsNoP = 0;
     size_t sclPerSlot = 0;
     // Host-side flags.
     std::vector<char> dirty;       // per node
@@ -298,6 +300,9 @@ struct PruneArgs {
     int nWOps, nPre, nLds;   // nLds: LDS slots per wave
     int tipLds;              // 1: the tile's tip codes are staged in LDS
     int nTax;
+    // P^T of every child in program order, [nKidTot][K][16] (k_pgather)
+    const void *PP;
+    int nKidTot, pLds;
 };
 
 // Kernel launchers (p4g_kernels.hip).
@@ -322,6 +327,9 @@ bool pruneUsesWave(int dim, int K, int precision);
 size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int precision);
 int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision);
 hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s);
+// P^T of the program's children in program order: PP[k][j] = PT[kidP[j]][k]
+hipError_t launchPgather(const void *PT, const int *kidP, int nKidTot, int K, void *PP, int precision,
+                         hipStream_t s);
 hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, int precision, size_t lds,
                                 hipStream_t s);
 

@@ -244,20 +244,21 @@ template <int SC, typename T>
 __device__ __forceinline__ void leafVec(const T *Pk, unsigned code, int S, const uint64_t *amb, T *v)
 {
     constexpr int SM = SC ? SC : MAX_DIM;
+    constexpr int UR = SC ? SM : 1;  // the generic path keeps its loops rolled
     if (code < (unsigned)S) {
-#pragma unroll
+#pragma unroll UR
         for (int s = 0; s < SM; s++)
             if (s < S) v[s] = Pk[code * S + s];
     } else if (code == TIP_ALL) {
-#pragma unroll
+#pragma unroll UR
         for (int s = 0; s < SM; s++) v[s] = T(1);
     } else {
         const uint64_t m = amb[code - S];
-#pragma unroll
+#pragma unroll UR
         for (int s = 0; s < SM; s++) v[s] = T(0);
         for (int j = 0; j < S; j++)
             if ((m >> j) & 1ull) {
-#pragma unroll
+#pragma unroll UR
                 for (int s = 0; s < SM; s++)
                     if (s < S) v[s] += Pk[j * S + s];
             }
@@ -270,12 +271,13 @@ template <int SC, typename T>
 __device__ __forceinline__ void internalVec(const T *Pk, const T *x, int S, T *v)
 {
     constexpr int SM = SC ? SC : MAX_DIM;
-#pragma unroll
+    constexpr int UR = SC ? SM : 1;  // the generic path keeps its loops rolled
+#pragma unroll UR
     for (int s = 0; s < SM; s++) v[s] = T(0);
-#pragma unroll
+#pragma unroll UR
     for (int j = 0; j < SM; j++) {
         if (j < S) {
-#pragma unroll
+#pragma unroll UR
             for (int s = 0; s < SM; s++)
                 if (s < S) v[s] += x[j] * Pk[j * S + s];
         }
@@ -289,8 +291,9 @@ template <int SC, typename T>
 __device__ __forceinline__ void rescale(T *acc, int &e, int S, int Kp)
 {
     constexpr int SM = SC ? SC : MAX_DIM;
+    constexpr int UR = SC ? SM : 1;  // the generic path keeps its loops rolled
     T m = T(0);
-#pragma unroll
+#pragma unroll UR
     for (int s = 0; s < SM; s++)
         if (s < S) m = fmax(m, acc[s]);
     m = groupMax(m, Kp);
@@ -298,7 +301,7 @@ __device__ __forceinline__ void rescale(T *acc, int &e, int S, int Kp)
         int ex;
         frexp(m, &ex);
         const int E = -ex;
-#pragma unroll
+#pragma unroll UR
         for (int s = 0; s < SM; s++)
             if (s < S) acc[s] = ldexp(acc[s], E);
         e += E;
@@ -438,6 +441,7 @@ template <int SC, typename T, bool SCALE, bool PLDS, int G>
 __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk, T *sP)
 {
     constexpr int SM = SC ? SC : MAX_DIM;
+    constexpr int UR = SC ? SM : 1;  // the generic path keeps its loops rolled
     const int S = SC ? SC : a.dim;
     const int K = a.K;
     const int lk = a.log2Kp;
@@ -461,7 +465,7 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
         const int total = a.nKids * cpk;
         for (int c0 = tid; c0 < total; c0 += PRUNE_THREADS * 8) {
             float4 buf[8];
-#pragma unroll
+#pragma unroll UR
             for (int u = 0; u < 8; u++) {
                 const int c = c0 + u * PRUNE_THREADS;
                 if (c < total) {
@@ -470,7 +474,7 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
                     buf[u] = src[c - kid * cpk];
                 }
             }
-#pragma unroll
+#pragma unroll UR
             for (int u = 0; u < 8; u++) {
                 const int c = c0 + u * PRUNE_THREADS;
                 if (c < total) reinterpret_cast<float4 *>(sP)[c] = buf[u];
@@ -491,12 +495,12 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
             T xv[G][2][SM];
             unsigned code[G][2];
             int ek[G];
-#pragma unroll
+#pragma unroll UR
             for (int j = 0; j < G; j++) {
                 ek[j] = 0;
                 if (j < gr.nOps) {
                     const OpRec op = a.ops[gr.opStart + j];
-#pragma unroll
+#pragma unroll UR
                     for (int c = 0; c < 2; c++) {
                         const KidRec kd = a.kids[op.kidStart + c];
                         if (kd.kind == 0) {
@@ -504,7 +508,7 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
                         } else {
                             code[j][c] = 0u;
                             const T *x = cl + (((size_t)kd.idx * nPatPad + p) * K + kk) * S;
-#pragma unroll
+#pragma unroll UR
                             for (int s = 0; s < SM; s++)
                                 if (s < S) xv[j][c][s] = x[s];
                             if (SCALE) ek[j] += a.scl[(size_t)kd.idx * nPatPad + p];
@@ -512,12 +516,12 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
                     }
                 }
             }
-#pragma unroll
+#pragma unroll UR
             for (int j = 0; j < G; j++) {
                 if (j < gr.nOps) {
                     const OpRec op = a.ops[gr.opStart + j];
                     T acc[SM], v[SM];
-#pragma unroll
+#pragma unroll UR
                     for (int c = 0; c < 2; c++) {
                         const KidRec kd = a.kids[op.kidStart + c];
                         const T *Pk = Pof(op.kidStart + c, kd.pnode);
@@ -526,10 +530,10 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
                         else
                             internalVec<SC, T>(Pk, xv[j][c], S, v);
                         if (c == 0) {
-#pragma unroll
+#pragma unroll UR
                             for (int s = 0; s < SM; s++) acc[s] = v[s];
                         } else {
-#pragma unroll
+#pragma unroll UR
                             for (int s = 0; s < SM; s++) acc[s] *= v[s];
                         }
                     }
@@ -537,7 +541,7 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
                     if (SCALE) rescale<SC, T>(acc, e, S, Kp);
                     if (catOn) {
                         T *dst = cl + (((size_t)op.dst * nPatPad + p) * K + kk) * S;
-#pragma unroll
+#pragma unroll UR
                         for (int s = 0; s < SM; s++)
                             if (s < S) dst[s] = acc[s];
                     }
@@ -564,10 +568,10 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
                 if (SCALE) e += a.scl[(size_t)kd.idx * nPatPad + p];
             }
             if (c == 0) {
-#pragma unroll
+#pragma unroll UR
                 for (int s = 0; s < SM; s++) acc[s] = v[s];
             } else if (!skip) {
-#pragma unroll
+#pragma unroll UR
                 for (int s = 0; s < SM; s++) acc[s] *= v[s];
             }
         }
@@ -575,7 +579,7 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
         if (!(op.flags & OP_ROOT)) {
             if (catOn) {
                 T *dst = cl + (((size_t)op.dst * nPatPad + p) * K + kk) * S;
-#pragma unroll
+#pragma unroll UR
                 for (int s = 0; s < SM; s++)
                     if (s < S) dst[s] = acc[s];
             }
@@ -594,11 +598,11 @@ __device__ __forceinline__ void pruneBody(const PruneArgs &a, int blk, int nBlk,
                 m = ~0ull;
             else
                 m = a.ambMask[code - S];
-#pragma unroll
+#pragma unroll UR
             for (int s = 0; s < SM; s++)
                 if (s < S && ((m >> s) & 1ull)) Lk += pi[s] * (double)acc[s];
         } else {
-#pragma unroll
+#pragma unroll UR
             for (int s = 0; s < SM; s++)
                 if (s < S) Lk += pi[s] * (double)acc[s];
         }
@@ -637,8 +641,8 @@ static int g_tuneG[2] = {4, 4};      // [precision]
 static int g_tuneLds[2] = {1, 1};
 static int g_wave[2] = {1, 1};       // 1: 4-state parts use k_pruneWave
 
-// G = 0 selects the wave-per-category kernel for 4-state parts; G = 1, 2,
-// 4 or 8 the grouped kernel with that many ops per group.
+// G = 0 selects the wave-per-category kernel for 4-state parts; G = 1 or 4
+// the grouped kernel with that many ops per group (other values: 4).
 void setPruneTuning(int precision, int G, int lds)
 {
     if (precision < 0 || precision > 1) return;
@@ -647,7 +651,7 @@ void setPruneTuning(int precision, int G, int lds)
         return;
     }
     g_wave[precision] = 0;
-    if (G == 1 || G == 2 || G == 4 || G == 8) g_tuneG[precision] = G;
+    g_tuneG[precision] = G == 1 ? 1 : 4;   // the grouped kernel is built for 1 and 4
     g_tuneLds[precision] = lds ? 1 : 0;
 }
 
@@ -670,8 +674,6 @@ static hipError_t launchPruneP(const PruneArgs &a, size_t lds, int nBlocks, hipS
     if (SC != 4) return launchPruneG<SC, T, SCALE, 1>(a, lds, nBlocks, s);
     switch (pruneGroupWidthFor(4, SCALE ? FP32 : FP64)) {
     case 1: return launchPruneG<SC, T, SCALE, 1>(a, lds, nBlocks, s);
-    case 2: return launchPruneG<SC, T, SCALE, 2>(a, lds, nBlocks, s);
-    case 8: return launchPruneG<SC, T, SCALE, 8>(a, lds, nBlocks, s);
     default: return launchPruneG<SC, T, SCALE, 4>(a, lds, nBlocks, s);
     }
 }
@@ -719,8 +721,6 @@ static hipError_t launchBatchP(const PruneArgs *d, const int *bi, int nBlocks, s
     if (SC != 4) return launchBatchG<SC, T, SCALE, 1>(d, bi, nBlocks, lds, s);
     switch (pruneGroupWidthFor(4, SCALE ? FP32 : FP64)) {
     case 1: return launchBatchG<SC, T, SCALE, 1>(d, bi, nBlocks, lds, s);
-    case 2: return launchBatchG<SC, T, SCALE, 2>(d, bi, nBlocks, lds, s);
-    case 8: return launchBatchG<SC, T, SCALE, 8>(d, bi, nBlocks, lds, s);
     default: return launchBatchG<SC, T, SCALE, 4>(d, bi, nBlocks, lds, s);
     }
 }
@@ -1099,30 +1099,63 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
 
     cint_t *ops = (cint_t *)(a.wops);
     cint_t *side = (cint_t *)(a.wkids);
-    cT *PT = (cT *)(a.PT);
+    // P^T of the program's children in program order, [kid][K][16]: the
+    // address of an op's P follows from the running child count, so the
+    // next op's record and P are loaded while the current op computes
+    cT *PP = (cT *)(a.PP);
+    const int nW = a.nWOps;
     double contrib = 0.0;
     int bad = 0;
-    for (int i = 0; i < a.nWOps; i++) {
+    int kb = 0;
+    int nNext = nW > 0 ? ops[3] : 0;
+    T Pn0[16], Pn1[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        Pn0[j] = nNext > 0 ? PP[(size_t)cat * 16 + j] : T(0);
+        Pn1[j] = nNext > 1 ? PP[((size_t)K + cat) * 16 + j] : T(0);
+    }
+    for (int i = 0; i < nW; i++) {
         cint_t *op = ops + 16 * i;
-        const int nk = op[3];
+        const int nk = nNext;
+        const int kbC = kb;
+        T P0[16], P1[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            P0[j] = Pn0[j];
+            P1[j] = Pn1[j];
+        }
+        kb += nk;
+        if (i + 1 < nW) {
+            nNext = ops[16 * (i + 1) + 3];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                Pn0[j] = PP[((size_t)kb * K + cat) * 16 + j];
+                Pn1[j] = nNext > 1 ? PP[((size_t)(kb + 1) * K + cat) * 16 + j] : T(0);
+            }
+        }
         T acc[4];
         int e = 0;
         for (int c = 0; c < nk; c++) {
-            int kind, idx, pn;
+            int kind, idx;
             if (c < 3) {
                 kind = op[5 + c];
                 idx = op[8 + c];
-                pn = op[11 + c];
             } else {
                 cint_t *kr = side + 4 * (op[4] + c - 3);
                 kind = kr[0];
                 idx = kr[1];
-                pn = kr[2];
             }
-            cT *Pk = PT + ((size_t)pn * K + cat) * 16;
             T Pl[16];
+            if (c == 0) {
 #pragma unroll
-            for (int j = 0; j < 16; j++) Pl[j] = Pk[j];
+                for (int j = 0; j < 16; j++) Pl[j] = P0[j];
+            } else if (c == 1) {
+#pragma unroll
+                for (int j = 0; j < 16; j++) Pl[j] = P1[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 16; j++) Pl[j] = PP[((size_t)(kbC + c) * K + cat) * 16 + j];
+            }
             T v[4];
             bool skip = false;
             if (kind == WK_TIP) {
@@ -1227,6 +1260,31 @@ __global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneWaveBatch(const PruneA
     pruneWaveBody<T, SCALE>(a, local, a.nPatPad / 64);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_pgather(const T *PT, const int *kidP, int nKidTot, int K, T *PP)
+{
+    // PP[j][k][e] = PT[kidP[j]][k][e]: P^T of the program's children in
+    // program order
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int per = K * 16;
+    if (i >= nKidTot * per) return;
+    const int j = i / per, r = i - j * per;
+    PP[i] = PT[(size_t)kidP[j] * per + r];
+}
+
+hipError_t launchPgather(const void *PT, const int *kidP, int nKidTot, int K, void *PP, int precision,
+                         hipStream_t s)
+{
+    const int n = K * nKidTot * 16;
+    const dim3 grid((n + 255) / 256);
+    if (precision == FP32)
+        hipLaunchKernelGGL(k_pgather<float>, grid, dim3(256), 0, s, (const float *)PT, kidP, nKidTot, K, (float *)PP);
+    else
+        hipLaunchKernelGGL(k_pgather<double>, grid, dim3(256), 0, s, (const double *)PT, kidP, nKidTot, K,
+                           (double *)PP);
+    return hipGetLastError();
+}
+
 bool pruneUsesWave(int dim, int K, int precision)
 {
     return dim == 4 && K >= 1 && K <= WAVE_MAX_K && g_wave[precision & 1];
@@ -1255,6 +1313,7 @@ int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision)
 
 hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s)
 {
+    // lds <= WAVE_LDS_BUDGET, within the default per-workgroup limit
     // lds <= WAVE_LDS_BUDGET, within the default per-workgroup limit
     const dim3 block(64 * a.K);
     if (precision == FP32)

@@ -42,7 +42,7 @@ def autotune(workload="het64", reps=30, seed=1, write=True, verbose=True):
         res[(0, 1)] = time_variant(tree, reps)
         if verbose:
             print("%s wave  %.1f us" % (prec, res[(0, 1)]), flush=True)
-        for G in (1, 2, 4, 8):
+        for G in (1, 4):
             for lds in (0, 1):
                 pf.setPruneTuning(prec == "fp32", G, lds)
                 res[(G, lds)] = time_variant(tree, reps)
