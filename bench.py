@@ -126,12 +126,15 @@ def main():
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
     if args.calibrate_copy:
+        # an elementwise kernel of known traffic (256 MiB read, 256 MiB
+        # written) for scaling the FETCH_SIZE / WRITE_SIZE counters; a
+        # device-to-device copy_ may go to a DMA engine the counters miss
         import torch
         n = 256 << 20
-        a = torch.empty(n // 4, dtype=torch.float32, device="cuda")
+        a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         for _ in range(5):
-            b.copy_(a)
+            torch.add(a, 1.0, out=b)
         torch.cuda.synchronize()
 
     out = {

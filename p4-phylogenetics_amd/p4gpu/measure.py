@@ -1,0 +1,116 @@
+"""Measurements beside the headline benchmark (bench.py measures het64).
+
+  prot128   128 taxa, 20,000 sites, 20 states, 2 comps, G4: one full FP32
+            evaluation on the MFMA kernel per step; reports the kernel's
+            device time, its HBM fraction (algorithmic bytes 80,132 B per
+            pattern) and MFMA utilisation (algorithmic flops
+            K*2*S^2*E_int per pattern against 157.3 TFLOP/s FP32).
+  multi48   48 taxa, 24 partitions (16 DNA, 8 protein, sites log-uniform
+            in [200, 40,000]), 4 Metropolis-coupled chains: one unit of work
+            = the pending evaluations of 4 chains in one batched call
+            (p4g_batchLogLike); full recompute and dirty-path variants.
+
+    python -m p4gpu.measure prot128 [--steps 50]
+    python -m p4gpu.measure multi48 [--steps 50]
+"""
+import argparse
+import json
+import random
+import time
+
+from . import pf, synth
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def prot128(steps, warmup):
+    tree, data = synth.make_workload("prot128", seed=1, precision="fp32")
+    tree.calcLogLike(verbose=0)
+    c = tree.cTree
+    for _ in range(warmup):
+        pf.p4_setPrams(c, -1)
+        pf.p4_treeLogLike(c, 0)
+    pf.setKernelTiming(True)
+    kus = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pf.p4_setPrams(c, -1)
+        lnl = pf.p4_treeLogLike(c, 0)
+        kus.append(pf.lastKernelMicroseconds(c))
+    sec = (time.perf_counter() - t0) / steps
+    pf.setKernelTiming(False)
+    mp = tree.model.parts[0]
+    S, K = mp.dim, mp.nGammaCat
+    n = sum(1 for _ in tree.iterLeavesNoRoot())
+    nInt = sum(1 for _ in tree.iterInternals())
+    eInt = nInt - 1
+    P = data.parts[0].nPatterns
+    bytes_per = 2 * eInt * S * K * 4 + n + 4
+    flops_per = K * 2 * S * S * eInt
+    ku = sum(kus) / len(kus)
+    tree.setPrecision("fp64")
+    lnl64 = tree.calcLogLike(verbose=0)
+    return dict(workload="prot128: 128 taxa, 20,000 sites (%d patterns), 20 states, 2 comps, G4, FP32 MFMA" % P,
+                evals_per_s=1.0 / sec, kernel_us=ku,
+                hbm_GBs=bytes_per * P / (ku * 1e-6) / 1e9, hbm_frac=bytes_per * P / (ku * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                mfma_TFLOPs=flops_per * P / (ku * 1e-6) / 1e12,
+                mfma_util=flops_per * P / (ku * 1e-6) / 1e12 / FP32_PEAK_TFLOPS,
+                bytes_per_pattern=bytes_per, flops_per_pattern=flops_per,
+                lnL_fp32=lnl, lnL_fp64=lnl64, rel_err=abs(lnl - lnl64) / abs(lnl64))
+
+
+def multi48(steps, warmup):
+    base, data = synth.make_workload("multi48", seed=1, precision="fp32")
+    base.calcLogLike(verbose=0)
+    trees = [base] + [base.dupe() for _ in range(3)]
+    for t in trees[1:]:
+        t.precision = "fp32"
+        t.calcLogLike(verbose=0)
+    batch = pf.Batch([t.cTree for t in trees])
+    rng = random.Random(3)
+
+    def full():
+        for t in trees:
+            pf.p4_setPrams(t.cTree, -1)
+        return batch.logLikes()
+
+    def dirty():
+        # one branch-length change per chain, its path marked in every part
+        for t in trees:
+            nodes = [n for n in t.iterNodesNoRoot()]
+            n = nodes[rng.randrange(len(nodes))]
+            pf.p4_setBrLen(n.cNode, n.br.len * (1.0 + 0.01 * (rng.random() - 0.5)))
+            pf.p4_calculateBigPDecks(n.cNode)
+            m = n.parent
+            while m is not None:
+                for p in range(t.model.nParts):
+                    pf.p4_setConditionalLikelihoodsOfInternalNodePart(m.cNode, p)
+                m = m.parent
+        return batch.logLikes()
+
+    out = dict(workload="multi48: 48 taxa, 24 partitions (16 DNA + 8 protein, %d patterns), 4 chains, FP32"
+               % sum(p.nPatterns for p in data.parts))
+    for name, fn in (("full", full), ("dirty_path", dirty)):
+        for _ in range(warmup):
+            fn()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        sec = (time.perf_counter() - t0) / steps
+        out[name] = dict(units_per_s=1.0 / sec, ms_per_unit=sec * 1e3)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("what", choices=["prot128", "multi48"])
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    a = ap.parse_args(argv)
+    res = prot128(a.steps, a.warmup) if a.what == "prot128" else multi48(a.steps, a.warmup)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()
