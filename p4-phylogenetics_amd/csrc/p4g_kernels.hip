@@ -32,6 +32,23 @@ __device__ __forceinline__ int rrPlayer(int pos, int r, int n)
     return pos == 0 ? 0 : ((pos - 1 + r) % (n - 1)) + 1;
 }
 
+// 1/x and 1/sqrt(x) from the hardware estimates plus two Newton steps
+__device__ __forceinline__ double rcpN(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
+__device__ __forceinline__ double rsqN(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    y = y * fma(-0.5 * x * y, y, 1.5);
+    return y;
+}
+
 __device__ __forceinline__ double waveSum(double v)
 {
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
@@ -116,13 +133,20 @@ __global__ __launch_bounds__(64) void k_eig(EigArgs a)
                 if (q < S) {
                     double apq = A[p * n + q];
                     if (apq != 0.0) {
-                        double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+                        // the rotation angle needs no correctly rounded divide:
+                        // reciprocal and reciprocal square root estimates,
+                        // each refined by two Newton steps (full precision),
+                        // keep the dependent FP64 chain short
+                        const double theta = (A[q * n + q] - A[p * n + p]) * rcpN(2.0 * apq);
                         double t;
-                        if (fabs(theta) > 1e150)
-                            t = 0.5 / theta;
-                        else
-                            t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                        c = 1.0 / sqrt(t * t + 1.0);
+                        if (fabs(theta) > 1e150) {
+                            t = 0.5 * rcpN(theta);
+                        } else {
+                            const double q2 = theta * theta + 1.0;
+                            const double root = q2 * rsqN(q2);
+                            t = (theta >= 0 ? 1.0 : -1.0) * rcpN(fabs(theta) + root);
+                        }
+                        c = rsqN(t * t + 1.0);
                         s = t * c;
                     }
                 }
