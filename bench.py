@@ -115,7 +115,7 @@ def main():
     kname = "k_prune<4,float,true>"
     try:
         with open(pf.TUNING_PATH) as fh:
-            if int(json.load(fh)["fp32"]["opsPerGroup"]) == 0:
+            if int(json.load(fh)["fp32"]["opsPerGroup"]) <= 0:
                 kname = "k_pruneWave<float,true>"
     except (OSError, ValueError, KeyError):
         kname = "k_pruneWave<float,true>"

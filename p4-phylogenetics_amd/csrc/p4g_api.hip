@@ -1047,7 +1047,7 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
     fillPruneArgs(t, pNum, nOps, a, siteLikes);
     hipStream_t s = stream();
     const int nBlocks = tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision);
-    if (tp.wave) {
+    if (tp.wave && pruneWaveVectorP(t->precision)) {
         const int *kidP = a.wpre + 2 * tp.nPre;
         if (!hipOk(launchPgather(a.PT, kidP, tp.nKidTot, tp.K, tp.PP, t->precision, s), "launch P gather"))
             return false;

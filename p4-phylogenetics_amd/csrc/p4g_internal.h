@@ -324,6 +324,7 @@ hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, in
 // kernel: partials [slot][cat][pattern][state], FP32 exponents
 // [slot][cat][pattern]; 64 patterns per workgroup of 64*K threads.
 bool pruneUsesWave(int dim, int K, int precision);
+bool pruneWaveVectorP(int precision);   // the wave kernel reads the k_pgather P stream
 size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int precision);
 int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision);
 hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s);

@@ -664,14 +664,18 @@ __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneBatch(const PruneArgs *a
 static int g_tuneG[2] = {4, 4};      // [precision]
 static int g_tuneLds[2] = {1, 1};
 static int g_wave[2] = {1, 1};       // 1: 4-state parts use k_pruneWave
+static int g_waveVecP[2] = {0, 0};   // 1: its vector-prefetched P stream
 
-// G = 0 selects the wave-per-category kernel for 4-state parts; G = 1 or 4
-// the grouped kernel with that many ops per group (other values: 4).
+// G = 0 selects the wave-per-category kernel for 4-state parts with its
+// vector-prefetched P stream, G = -1 the same kernel with P by scalar
+// loads; G = 1 or 4 the grouped kernel with that many ops per group (other
+// values: 4).
 void setPruneTuning(int precision, int G, int lds)
 {
     if (precision < 0 || precision > 1) return;
-    if (G == 0) {
+    if (G == 0 || G == -1) {
         g_wave[precision] = 1;
+        g_waveVecP[precision] = G == 0 ? 1 : 0;
         return;
     }
     g_wave[precision] = 0;
@@ -1066,7 +1070,7 @@ __device__ __forceinline__ void finishSiteSumWave(const PruneArgs &a, double con
     }
 }
 
-template <typename T, bool SCALE>
+template <typename T, bool SCALE, bool PV>
 __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int nTiles)
 {
     typedef typename Vec4T<T>::type V4;
@@ -1081,10 +1085,34 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
     const size_t nPP = (size_t)a.nPatPad;
     const int nL = a.nLds;
     const int tipBytes = a.tipLds ? ((a.nTax * 64 + 15) & ~15) : 0;
-    const unsigned char *sTip = smemW;
-    V4 *slot = reinterpret_cast<V4 *>(smemW + tipBytes) + (size_t)cat * nL * 64 + lane;
-    int *sexp = reinterpret_cast<int *>(smemW + tipBytes + sizeof(V4) * (size_t)K * nL * 64) +
-                (size_t)cat * nL * 64 + lane;
+    // explicit address spaces keep LDS and HBM reads of a child as separate
+    // ds_ / global_ instructions (one merged generic load would wait on
+    // both memory counters at every child)
+    typedef __attribute__((address_space(3))) T lT;
+    typedef __attribute__((address_space(3))) int lint;
+    typedef __attribute__((address_space(3))) const unsigned char lu8;
+    typedef __attribute__((address_space(1))) const T gT;
+    typedef __attribute__((address_space(1))) const int gint;
+    typedef __attribute__((address_space(1))) const unsigned char gu8;
+    lu8 *sTip = (lu8 *)(smemW);
+    lT *slotT = (lT *)(smemW + tipBytes) + ((size_t)cat * nL * 64 + lane) * 4;   // slot s: slotT[s * 256 + 0..3]
+    auto ldS = [&](int sl) {
+        V4 x;
+        x.x = slotT[sl * 256];
+        x.y = slotT[sl * 256 + 1];
+        x.z = slotT[sl * 256 + 2];
+        x.w = slotT[sl * 256 + 3];
+        return x;
+    };
+    auto stS = [&](int sl, const V4 &x) {
+        slotT[sl * 256] = x.x;
+        slotT[sl * 256 + 1] = x.y;
+        slotT[sl * 256 + 2] = x.z;
+        slotT[sl * 256 + 3] = x.w;
+    };
+    lint *sexp = (lint *)(smemW + tipBytes + sizeof(V4) * (size_t)K * nL * 64) + (size_t)cat * nL * 64 + lane;
+    gu8 *gTips = (gu8 *)(a.tips);
+    gint *gScl = (gint *)(a.scl);
     if (a.tipLds) {
         // the tile's tip codes, [taxon][64 patterns], 4 bytes per thread
         const int words = a.nTax * 16;
@@ -1095,107 +1123,174 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
         }
         __syncthreads();
     }
-    const V4 *clr = reinterpret_cast<const V4 *>(a.cl);
+    gT *clr = (gT *)(a.cl);
     V4 *clw = reinterpret_cast<V4 *>(a.cl);
+    auto ldG = [&](size_t o) {
+        V4 x;
+        x.x = clr[4 * o];
+        x.y = clr[4 * o + 1];
+        x.z = clr[4 * o + 2];
+        x.w = clr[4 * o + 3];
+        return x;
+    };
     const size_t catOff = (size_t)cat * nPP + p;   // element of slot s: s*K*nPP + catOff
     const size_t slotStride = (size_t)K * nPP;
 
     // clean children read by the program: loads issued in pairs before
     // their LDS stores
-    cint_t *pre = (cint_t *)(a.wpre);
+    const int *pre = a.wpre;
     for (int i = 0; i < a.nPre; i += 2) {
         const bool two = i + 1 < a.nPre;
-        const size_t o0 = (size_t)pre[2 * i] * slotStride + catOff;
-        const size_t o1 = two ? (size_t)pre[2 * i + 2] * slotStride + catOff : o0;
-        const V4 x0 = clr[o0];
-        const V4 x1 = clr[o1];
-        const int e0 = SCALE ? a.scl[o0] : 0;
-        const int e1 = SCALE ? a.scl[o1] : 0;
-        const int l0 = pre[2 * i + 1];
-        slot[l0 * 64] = x0;
+        const size_t o0 = (size_t)__builtin_amdgcn_readfirstlane(pre[2 * i]) * slotStride + catOff;
+        const size_t o1 = two ? (size_t)__builtin_amdgcn_readfirstlane(pre[2 * i + 2]) * slotStride + catOff : o0;
+        const V4 x0 = ldG(o0);
+        const V4 x1 = ldG(o1);
+        const int e0 = SCALE ? gScl[o0] : 0;
+        const int e1 = SCALE ? gScl[o1] : 0;
+        const int l0 = __builtin_amdgcn_readfirstlane(pre[2 * i + 1]);
+        stS(l0, x0);
         if (SCALE) sexp[l0 * 64] = e0;
         if (two) {
-            const int l1 = pre[2 * i + 3];
-            slot[l1 * 64] = x1;
+            const int l1 = __builtin_amdgcn_readfirstlane(pre[2 * i + 3]);
+            stS(l1, x1);
             if (SCALE) sexp[l1 * 64] = e1;
         }
     }
 
-    cint_t *ops = (cint_t *)(a.wops);
-    cint_t *side = (cint_t *)(a.wkids);
-    // P^T of the program's children in program order, [kid][K][16]: the
-    // address of an op's P follows from the running child count, so the
-    // next op's record and P are loaded while the current op computes
-    cT *PP = (cT *)(a.PP);
+    // Program records and P^T.  PV = 1: records and the program-order P^T
+    // stream ([kid][K][16], k_pgather) come through vector loads from
+    // wave-uniform addresses, one op ahead of use (scalar loads share their
+    // wait counter with the slot file's LDS traffic).  PV = 0: records and
+    // P^T (indexed by the child's node) through scalar loads when used.
+    // Both compute the same values; the autotuner picks the faster.
+    const int4 *ops4 = reinterpret_cast<const int4 *>(a.wops);
+    cint_t *opsS = (cint_t *)(a.wops);
+    const int *side = reinterpret_cast<const int *>(a.wkids);
+    const T *PPg = reinterpret_cast<const T *>(a.PP);
+    cT *PTs = (cT *)(a.PT);
     const int nW = a.nWOps;
+    const int nKT = a.nKidTot;
     double contrib = 0.0;
     int bad = 0;
     int kb = 0;
-    int nNext = nW > 0 ? ops[3] : 0;
+    int4 rn0 = make_int4(0, 0, 0, 0), rn1 = rn0, rn2 = rn0;
     T Pn0[16], Pn1[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-        Pn0[j] = nNext > 0 ? PP[(size_t)cat * 16 + j] : T(0);
-        Pn1[j] = nNext > 1 ? PP[((size_t)K + cat) * 16 + j] : T(0);
-    }
-    for (int i = 0; i < nW; i++) {
-        cint_t *op = ops + 16 * i;
-        const int nk = nNext;
-        const int kbC = kb;
-        T P0[16], P1[16];
+    if (PV) {
+        rn0 = ops4[0];
+        rn1 = ops4[1];
+        rn2 = ops4[2];
 #pragma unroll
         for (int j = 0; j < 16; j++) {
-            P0[j] = Pn0[j];
-            P1[j] = Pn1[j];
+            Pn0[j] = PPg[(size_t)cat * 16 + j];
+            Pn1[j] = nKT > 1 ? PPg[((size_t)K + cat) * 16 + j] : T(0);
         }
-        kb += nk;
-        if (i + 1 < nW) {
-            nNext = ops[16 * (i + 1) + 3];
+    }
+    for (int i = 0; i < nW; i++) {
+        int nk, opDst, opLds, opFlags, kidStart;
+        int4 r1, r2;
+        T P0[16], P1[16];
+        int kbC = kb;
+        if (PV) {
+            const int4 r0 = rn0;
+            r1 = rn1;
+            r2 = rn2;
 #pragma unroll
             for (int j = 0; j < 16; j++) {
-                Pn0[j] = PP[((size_t)kb * K + cat) * 16 + j];
-                Pn1[j] = nNext > 1 ? PP[((size_t)(kb + 1) * K + cat) * 16 + j] : T(0);
+                P0[j] = Pn0[j];
+                P1[j] = Pn1[j];
             }
+            // record: [dst, ldsDst, flags, nKids] [kidStart, kind0..2] [idx0..2, pnode0] ...
+            nk = __builtin_amdgcn_readfirstlane(r0.w);
+            kb += nk;
+            if (i + 1 < nW) {
+                rn0 = ops4[4 * (i + 1)];
+                rn1 = ops4[4 * (i + 1) + 1];
+                rn2 = ops4[4 * (i + 1) + 2];
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    Pn0[j] = PPg[((size_t)kb * K + cat) * 16 + j];
+                    Pn1[j] = kb + 1 < nKT ? PPg[((size_t)(kb + 1) * K + cat) * 16 + j] : T(0);
+                }
+            }
+            opDst = __builtin_amdgcn_readfirstlane(r0.x);
+            opLds = __builtin_amdgcn_readfirstlane(r0.y);
+            opFlags = __builtin_amdgcn_readfirstlane(r0.z);
+            kidStart = __builtin_amdgcn_readfirstlane(r1.x);
+        } else {
+            cint_t *op = opsS + 16 * i;
+            nk = op[3];
+            opDst = op[0];
+            opLds = op[1];
+            opFlags = op[2];
+            kidStart = op[4];
         }
         T acc[4];
         int e = 0;
         for (int c = 0; c < nk; c++) {
             int kind, idx;
-            if (c < 3) {
-                kind = op[5 + c];
-                idx = op[8 + c];
-            } else {
-                cint_t *kr = side + 4 * (op[4] + c - 3);
-                kind = kr[0];
-                idx = kr[1];
-            }
             T Pl[16];
-            if (c == 0) {
+            if (PV) {
+                if (c == 0) {
+                    kind = __builtin_amdgcn_readfirstlane(r1.y);
+                    idx = __builtin_amdgcn_readfirstlane(r2.x);
+                } else if (c == 1) {
+                    kind = __builtin_amdgcn_readfirstlane(r1.z);
+                    idx = __builtin_amdgcn_readfirstlane(r2.y);
+                } else if (c == 2) {
+                    kind = __builtin_amdgcn_readfirstlane(r1.w);
+                    idx = __builtin_amdgcn_readfirstlane(r2.z);
+                } else {
+                    const int *kr = side + 4 * (kidStart + c - 3);
+                    kind = __builtin_amdgcn_readfirstlane(kr[0]);
+                    idx = __builtin_amdgcn_readfirstlane(kr[1]);
+                }
+                if (c == 0) {
 #pragma unroll
-                for (int j = 0; j < 16; j++) Pl[j] = P0[j];
-            } else if (c == 1) {
+                    for (int j = 0; j < 16; j++) Pl[j] = P0[j];
+                } else if (c == 1) {
 #pragma unroll
-                for (int j = 0; j < 16; j++) Pl[j] = P1[j];
+                    for (int j = 0; j < 16; j++) Pl[j] = P1[j];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 16; j++) Pl[j] = PPg[((size_t)(kbC + c) * K + cat) * 16 + j];
+                }
             } else {
+                cint_t *op = opsS + 16 * i;
+                int pn;
+                if (c < 3) {
+                    kind = op[5 + c];
+                    idx = op[8 + c];
+                    pn = op[11 + c];
+                } else {
+                    cint_t *kr = (cint_t *)(side) + 4 * (kidStart + c - 3);
+                    kind = kr[0];
+                    idx = kr[1];
+                    pn = kr[2];
+                }
+                cT *Pk = PTs + ((size_t)pn * K + cat) * 16;
 #pragma unroll
-                for (int j = 0; j < 16; j++) Pl[j] = PP[((size_t)(kbC + c) * K + cat) * 16 + j];
+                for (int j = 0; j < 16; j++) Pl[j] = Pk[j];
             }
             T v[4];
             bool skip = false;
             if (kind == WK_TIP) {
-                const unsigned code = a.tipLds ? sTip[idx * 64 + lane] : a.tips[(size_t)idx * nPP + p];
+                unsigned code;
+                if (a.tipLds)
+                    code = sTip[idx * 64 + lane];
+                else
+                    code = gTips[(size_t)idx * nPP + p];
                 leafVec4<T>(Pl, code, a.ambMask, v);
                 skip = (code == TIP_ALL) && c != 0;
             } else {
                 V4 x;
                 int ex = 0;
                 if (kind == WK_LDS) {
-                    x = slot[idx * 64];
+                    x = ldS(idx);
                     if (SCALE) ex = sexp[idx * 64];
                 } else {
                     const size_t o = (size_t)idx * slotStride + catOff;
-                    x = clr[o];
-                    if (SCALE) ex = a.scl[o];
+                    x = ldG(o);
+                    if (SCALE) ex = gScl[o];
                 }
                 const T xv[4] = {x.x, x.y, x.z, x.w};
                 internalVec<4, T>(Pl, xv, 4, v);
@@ -1220,18 +1315,18 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
                 e += E;
             }
         }
-        if (!(op[2] & OP_ROOT)) {
+        if (!(opFlags & OP_ROOT)) {
             V4 r;
             r.x = acc[0];
             r.y = acc[1];
             r.z = acc[2];
             r.w = acc[3];
-            const size_t o = (size_t)op[0] * slotStride + catOff;
+            const size_t o = (size_t)opDst * slotStride + catOff;
             clw[o] = r;
             if (SCALE) a.scl[o] = e;
-            const int ld = op[1];
+            const int ld = opLds;
             if (ld >= 0) {
-                slot[ld * 64] = r;
+                stS(ld, r);
                 if (SCALE) sexp[ld * 64] = e;
             }
             continue;
@@ -1269,10 +1364,10 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
     if (cat == 0) finishSiteSumWave(a, contrib, bad, tile, nTiles);
 }
 
-template <typename T, bool SCALE>
+template <typename T, bool SCALE, bool PV>
 __global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneWave(PruneArgs a)
 {
-    pruneWaveBody<T, SCALE>(a, blockIdx.x, gridDim.x);
+    pruneWaveBody<T, SCALE, PV>(a, blockIdx.x, gridDim.x);
 }
 
 template <typename T, bool SCALE>
@@ -1281,7 +1376,7 @@ __global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneWaveBatch(const PruneA
     const int item = blockItem[2 * blockIdx.x];
     const int local = blockItem[2 * blockIdx.x + 1];
     const PruneArgs a = args[item];
-    pruneWaveBody<T, SCALE>(a, local, a.nPatPad / 64);
+    pruneWaveBody<T, SCALE, true>(a, local, a.nPatPad / 64);
 }
 
 template <typename T>
@@ -1308,6 +1403,8 @@ hipError_t launchPgather(const void *PT, const int *kidP, int nKidTot, int K, vo
                            (double *)PP);
     return hipGetLastError();
 }
+
+bool pruneWaveVectorP(int precision) { return g_waveVecP[precision & 1] != 0; }
 
 bool pruneUsesWave(int dim, int K, int precision)
 {
@@ -1338,12 +1435,19 @@ int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision)
 hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s)
 {
     // lds <= WAVE_LDS_BUDGET, within the default per-workgroup limit
-    // lds <= WAVE_LDS_BUDGET, within the default per-workgroup limit
     const dim3 block(64 * a.K);
-    if (precision == FP32)
-        hipLaunchKernelGGL((k_pruneWave<float, true>), dim3(nBlocks), block, lds, s, a);
-    else
-        hipLaunchKernelGGL((k_pruneWave<double, false>), dim3(nBlocks), block, lds, s, a);
+    const bool pv = pruneWaveVectorP(precision);
+    if (precision == FP32) {
+        if (pv)
+            hipLaunchKernelGGL((k_pruneWave<float, true, true>), dim3(nBlocks), block, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_pruneWave<float, true, false>), dim3(nBlocks), block, lds, s, a);
+    } else {
+        if (pv)
+            hipLaunchKernelGGL((k_pruneWave<double, false, true>), dim3(nBlocks), block, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_pruneWave<double, false, false>), dim3(nBlocks), block, lds, s, a);
+    }
     return hipGetLastError();
 }
 

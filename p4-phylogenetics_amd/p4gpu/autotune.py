@@ -3,7 +3,8 @@
 Times the wave-per-category kernel and every (ops per group, P staged in
 LDS) variant of the grouped kernel on a workload, for FP32 and FP64
 partials, and writes the fastest to p4gpu/tuning.json, which the library
-applies when it is loaded (opsPerGroup 0 = the wave kernel).  The grouped
+applies when it is loaded (opsPerGroup 0 / -1 = the wave kernel with its
+P^T read by vector / scalar loads).  The grouped
 variants compute identical results; the wave kernel keeps FP32 scale
 exponents per category, which changes FP32 results only at rounding level.
 
@@ -38,10 +39,11 @@ def autotune(workload="het64", reps=30, seed=1, write=True, verbose=True):
         tree, _ = synth.make_workload(workload, seed=seed, precision=prec)
         tree.calcLogLike(verbose=0)
         res = {}
-        pf.setPruneTuning(prec == "fp32", 0, 1)
-        res[(0, 1)] = time_variant(tree, reps)
-        if verbose:
-            print("%s wave  %.1f us" % (prec, res[(0, 1)]), flush=True)
+        for G, label in ((-1, "wave, scalar P"), (0, "wave, vector P")):
+            pf.setPruneTuning(prec == "fp32", G, 1)
+            res[(G, 1)] = time_variant(tree, reps)
+            if verbose:
+                print("%s %s  %.1f us" % (prec, label, res[(G, 1)]), flush=True)
         for G in (1, 4):
             for lds in (0, 1):
                 pf.setPruneTuning(prec == "fp32", G, lds)
@@ -51,7 +53,8 @@ def autotune(workload="het64", reps=30, seed=1, write=True, verbose=True):
         (G, lds), best = min(res.items(), key=lambda kv: kv[1])
         out[prec] = {"opsPerGroup": G, "stagePInLds": lds, "kernel_us": best,
                      "workload": workload,
-                     "all": {("wave" if k[0] == 0 else "G%d_lds%d" % k): v for k, v in res.items()}}
+                     "all": {({0: "wave_vectorP", -1: "wave_scalarP"}.get(k[0]) or "G%d_lds%d" % k): v
+                             for k, v in res.items()}}
         pf.setPruneTuning(prec == "fp32", G, lds)
     if write:
         with open(pf.TUNING_PATH, "w") as fh:

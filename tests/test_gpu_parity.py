@@ -235,3 +235,21 @@ def test_no_gamma_single_category():
     tree.setPInvar(free=0, val=0.15)
     want = oracle.part_loglike(oracle.flat_from_tree(tree, 0))
     assert rel(tree.calcLogLike(verbose=0), want) < FP64_TOL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("fp64", FP64_TOL), ("fp32", FP32_TOL)])
+def test_wave_kernel_variants_match_golden(prec, tol):
+    """Both P-load variants of the wave kernel and the grouped kernel give
+    the reference lnL (the autotuner may pick any of them)."""
+    fx = load("het64_800")
+    fp32 = prec == "fp32"
+    try:
+        for G in (-1, 0, 4):
+            pf.setPruneTuning(fp32, G, 1)
+            t = tree_from_fixture(fx, prec)
+            v = t.calcLogLike(verbose=0)
+            assert rel(v, float(fx["lnL"])) < tol, (G, v, float(fx["lnL"]))
+    finally:
+        pf.setPruneTuning(fp32, -1, 1)   # the library default, then the tuned choice
+        pf._applyTuning()
