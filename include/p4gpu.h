@@ -223,9 +223,11 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL);
 void p4g_setKernelTiming(int on);
 /* Tuning of the 4-state pruning kernel (no counterpart in the reference;
  * set by the autotuner, p4gpu/autotune.py), for FP64 (fp32 = 0) or FP32
- * partials: opsPerGroup 0 or -1 selects the wave-per-category kernel (the
- * default) with its P^T read by vector loads from a program-order stream
- * (0) or by scalar loads (-1); 1 or 4 the grouped kernel with that many independent
+ * partials: opsPerGroup 0, -1 or -2 selects the wave-per-category kernel
+ * (the default) with its P^T read by vector loads from a program-order
+ * stream (0), by scalar loads (-1), or by scalar loads with the tree's
+ * subtrees split between two sets of waves (-2); 1 or 4 the grouped kernel
+ * with that many independent
  * binary ops per group, with the program's P rows staged in LDS if
  * stagePInLds.  Partials already computed are re-laid out (recomputed) at
  * the next evaluation when the choice changes the layout. */

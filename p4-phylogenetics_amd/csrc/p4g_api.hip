@@ -832,60 +832,115 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     const int maxSlots = pruneWaveMaxSlots(tp.K, tipLds, data->nTax, t->precision);
     std::vector<int> pos(nOps);
     for (int q = 0; q < nOps; q++) pos[order[q]] = q;
-    // pass 1: results only, to find their peak slot use
-    std::vector<int> ldsOf(nOps, -1);
-    auto simulate = [&](int nSlotsAvail, std::vector<int> &ldsOut, int &peak) {
-        std::vector<int> freeList;
-        for (int sl = nSlotsAvail - 1; sl >= 0; sl--) freeList.push_back(sl);
-        std::fill(ldsOut.begin(), ldsOut.end(), -1);
-        int used = 0;
-        peak = 0;
-        for (int q = 0; q < nOps; q++) {
-            const int i = order[q];
-            for (int c : progKids[i])
-                if (ldsOut[c] >= 0) {
-                    freeList.push_back(ldsOut[c]);
-                    used--;
-                }
-            if (parentOp[i] >= 0 && !freeList.empty()) {
-                ldsOut[i] = freeList.back();
-                freeList.pop_back();
-                used++;
-                peak = std::max(peak, used);
+    // Streams: with two, the subtrees under the root are split between two
+    // sets of category waves that run at the same time (half the node chain
+    // per wave); the root runs after a barrier on stream 0, reading stream
+    // 1's root children from junction slots in stream 0's slot file.
+    const int rootOp = order.back();
+    const int nStream = (pruneWaveStreams(t->precision) && tp.K * 2 <= WAVE_MAX_K && nOps > 3) ? 2 : 1;
+    // LDS budget shared by all slot files of the workgroup
+    const int maxSlotsS = pruneWaveMaxSlots(tp.K * nStream, tipLds, data->nTax, t->precision);
+    std::vector<int> top(nOps, -1);
+    for (int q = nOps - 1; q >= 0; q--) {
+        const int i = order[q];
+        if (i == rootOp) continue;
+        const int pa = parentOp[i];
+        top[i] = (pa < 0 || pa == rootOp) ? i : top[pa];
+    }
+    std::vector<int> streamOf(nOps, 0);
+    if (nStream == 2) {
+        std::vector<int> size(nOps, 0), tops;
+        for (int i = 0; i < nOps; i++)
+            if (i != rootOp) {
+                if (size[top[i]]++ == 0) tops.push_back(top[i]);
             }
+        std::stable_sort(tops.begin(), tops.end(), [&](int x, int y) { return size[x] > size[y]; });
+        int load[2] = {0, 0};
+        std::vector<int> streamOfTop(nOps, 0);
+        for (int tp_ : tops) {
+            const int st = load[1] < load[0] ? 1 : 0;
+            streamOfTop[tp_] = st;
+            load[st] += size[tp_];
         }
-    };
-    int peakRes = 0;
-    simulate(maxSlots, ldsOf, peakRes);
-    // clean internal children, by consumer position
-    struct Pre {
-        int q, op, c, cl;
-    };
-    std::vector<Pre> cleans;
+        for (int i = 0; i < nOps; i++)
+            if (i != rootOp) streamOf[i] = streamOfTop[top[i]];
+    }
+    std::vector<int> lists[2];
     for (int q = 0; q < nOps; q++) {
         const int i = order[q];
-        for (int c = 0; c < ops[i].nKids; c++) {
-            const KidRec &k = kids[ops[i].kidStart + c];
-            if (k.kind == 1 && !(opOfSlot[k.idx] >= 0 && opOfSlot[k.idx] < i)) cleans.push_back({q, i, c, k.idx});
+        if (i != rootOp) lists[streamOf[i]].push_back(i);
+    }
+    // slot use of each stream's results; stream 1's root children are left
+    // for the junction
+    std::vector<int> ldsOf(nOps, -1);
+    int peak[2] = {0, 0};
+    for (int st = 0; st < nStream; st++) {
+        std::vector<int> freeList;
+        for (int sl = maxSlotsS - 1; sl >= 0; sl--) freeList.push_back(sl);
+        int used = 0;
+        for (int i : lists[st]) {
+            for (int c : progKids[i])
+                if (ldsOf[c] >= 0) {
+                    freeList.push_back(ldsOf[c]);
+                    used--;
+                }
+            const bool junction = st == 1 && parentOp[i] == rootOp;
+            if (parentOp[i] >= 0 && !junction && !freeList.empty()) {
+                ldsOf[i] = freeList.back();
+                freeList.pop_back();
+                used++;
+                peak[st] = std::max(peak[st], used);
+            }
         }
     }
+    // junction slots follow stream 0's results: [peak0, peak0 + J)
+    int nJ = 0;
+    if (nStream == 2)
+        for (int i : lists[1])
+            if (parentOp[i] == rootOp && peak[0] + nJ < maxSlotsS) {
+                ldsOf[i] = -(2 + peak[0] + nJ);   // written into stream 0's file
+                nJ++;
+            }
+    // clean internal children, by consumer, per stream (the root's: stream 0)
+    struct Pre {
+        int st, op, c, cl;
+    };
+    std::vector<Pre> cleans;
+    for (int st = 0; st < nStream; st++) {
+        std::vector<int> lst = lists[st];
+        if (st == 0) lst.push_back(rootOp);
+        for (int i : lst)
+            for (int c = 0; c < ops[i].nKids; c++) {
+                const KidRec &k = kids[ops[i].kidStart + c];
+                if (k.kind == 1 && !(opOfSlot[k.idx] >= 0 && opOfSlot[k.idx] < i)) cleans.push_back({st, i, c, k.idx});
+            }
+    }
     // every preload is loaded at the start, so each takes a slot of its own
-    // above the results' [0, peakRes); the first consumers get them
-    const int nPreSlots = std::max(0, maxSlots - peakRes);
+    // above the results (and, in stream 0, the junction)
+    int nextPre[2] = {peak[0] + nJ, peak[1]};
     std::vector<int> preSlot(cleans.size(), -1);
     int nPre = 0;
-    for (size_t j = 0; j < cleans.size() && nPre < nPreSlots; j++) preSlot[j] = peakRes + nPre++;
-    const int nLds = peakRes + nPre;
-    // preloaded child lookup by (op, child index)
+    for (size_t j = 0; j < cleans.size(); j++) {
+        const int st = cleans[j].st;
+        if (nextPre[st] < maxSlotsS) {
+            preSlot[j] = nextPre[st]++;
+            nPre++;
+        }
+    }
+    const int nLds = std::max(nextPre[0], nStream == 2 ? nextPre[1] : 0);
     std::vector<std::vector<int>> preOf(nOps);
     for (int i = 0; i < nOps; i++) preOf[i].assign(ops[i].nKids, -1);
     for (size_t j = 0; j < cleans.size(); j++) preOf[cleans[j].op][cleans[j].c] = preSlot[j];
-    // emit
+    // emission order: stream 0, stream 1, the root
+    std::vector<int> emit = lists[0];
+    emit.insert(emit.end(), lists[1].begin(), lists[1].end());
+    emit.push_back(rootOp);
+    for (int q = 0; q < nOps; q++) pos[emit[q]] = q;
     int nSide = 0;
     for (int i = 0; i < nOps; i++) nSide += std::max(0, ops[i].nKids - 3);
     int nKidTot = 0;
     for (int i = 0; i < nOps; i++) nKidTot += ops[i].nKids;
-    const size_t words = (size_t)16 * nOps + 4 * (size_t)nSide + 2 * (size_t)nPre + (size_t)nKidTot;
+    const size_t words = (size_t)16 * nOps + 4 * (size_t)nSide + 3 * (size_t)nPre + (size_t)nKidTot;
     if ((int)words > tp.progCapWords) {
         setError("internal: node program larger than its staging area");
         return false;
@@ -893,10 +948,10 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     WaveOp *wo = reinterpret_cast<WaveOp *>(tp.hProg);
     WaveKid *wk = reinterpret_cast<WaveKid *>(tp.hProg + 16 * nOps);
     int *wp = tp.hProg + 16 * nOps + 4 * nSide;
-    int *kidP = wp + 2 * nPre;
+    int *kidP = wp + 3 * nPre;
     int side = 0, kb = 0;
     for (int q = 0; q < nOps; q++) {
-        const int i = order[q];
+        const int i = emit[q];
         WaveOp w{};
         w.dst = (ops[i].flags & OP_ROOT) ? -1 : ops[i].dst;
         w.ldsDst = ldsOf[i];
@@ -913,8 +968,16 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
             } else {
                 const int j = opOfSlot[k.idx];
                 if (j >= 0 && j < i && pos[j] < q) {
-                    kind = ldsOf[j] >= 0 ? WK_LDS : WK_GLOBAL;
-                    idx = ldsOf[j] >= 0 ? ldsOf[j] : k.idx;
+                    if (ldsOf[j] >= 0) {
+                        kind = WK_LDS;
+                        idx = ldsOf[j];
+                    } else if (ldsOf[j] <= -2) {   // junction: the root reads its own file
+                        kind = WK_LDS;
+                        idx = -ldsOf[j] - 2;
+                    } else {
+                        kind = WK_GLOBAL;
+                        idx = k.idx;
+                    }
                 } else if (preOf[i][c] >= 0) {
                     kind = WK_LDS;
                     idx = preOf[i][c];
@@ -937,8 +1000,9 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     int np = 0;
     for (size_t j = 0; j < cleans.size(); j++)
         if (preSlot[j] >= 0) {
-            wp[2 * np] = cleans[j].cl;
-            wp[2 * np + 1] = preSlot[j];
+            wp[3 * np] = cleans[j].cl;
+            wp[3 * np + 1] = preSlot[j];
+            wp[3 * np + 2] = cleans[j].st;
             np++;
         }
     tp.nWOps = nOps;
@@ -947,7 +1011,9 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     tp.nLds = nLds;
     tp.tipLds = tipLds;
     tp.nKidTot = nKidTot;
-    tp.waveLdsNoP = pruneWaveLdsBytes(tp.K, nLds, tipLds, data->nTax, t->precision);
+    tp.nStream = nStream;
+    tp.nS0 = (int)lists[0].size();
+    tp.waveLdsNoP = pruneWaveLdsBytes(tp.K * nStream, nLds, tipLds, data->nTax, t->precision);
     tp.pLds = 0;
     tp.waveLds = tp.waveLdsNoP;
     tp.progWords = (int)words;
@@ -994,6 +1060,8 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
         a.wops = reinterpret_cast<const WaveOp *>(prog);
         a.wkids = reinterpret_cast<const WaveKid *>(prog + 16 * tp.nWOps);
         a.wpre = prog + 16 * tp.nWOps + 4 * tp.nWKids;
+        a.nStream = tp.nStream;
+        a.nS0 = tp.nS0;
         a.nWOps = tp.nWOps;
         a.nPre = tp.nPre;
         a.nLds = tp.nLds;
@@ -1048,7 +1116,7 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
     hipStream_t s = stream();
     const int nBlocks = tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision);
     if (tp.wave && pruneWaveVectorP(t->precision)) {
-        const int *kidP = a.wpre + 2 * tp.nPre;
+        const int *kidP = a.wpre + 3 * tp.nPre;
         if (!hipOk(launchPgather(a.PT, kidP, tp.nKidTot, tp.K, tp.PP, t->precision, s), "launch P gather"))
             return false;
     }
@@ -2070,7 +2138,8 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         const int prec = items[i].t->precision;
         const int kc = pruneUsesMfma(mp->dim, mp->nCat, prec) ? (mp->nCat <= 1 ? 1 : mp->nCat <= 2 ? 2 : mp->nCat <= 4 ? 4 : 8) : 0;
         // the wave kernel's workgroup size is 64 * K: one launch per K
-        const int wk = items[i].t->tp[items[i].p].wave ? 16 + mp->nCat : 0;
+        const TreePart &tq = items[i].t->tp[items[i].p];
+        const int wk = tq.wave ? 16 * tq.nStream + mp->nCat : 0;
         return ((mp->dim * 2 + prec) * 16 + kc) * 32 + wk;
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
@@ -2086,7 +2155,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     std::vector<std::pair<int, int>> launches;  // (first block, count) per group
     std::vector<int> groupDim, groupPrec;
     std::vector<size_t> groupLds;
-    std::vector<int> groupK, groupWave;
+    std::vector<int> groupK, groupWave, groupStreams;
     while (j < nItems) {
         const int k0 = key(order[j]);
         const int first = blk;
@@ -2106,6 +2175,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         groupPrec.push_back(items[order[j]].t->precision);
         groupK.push_back(items[order[j]].t->model->parts[items[order[j]].p]->nCat);
         groupWave.push_back(items[order[j]].t->tp[items[order[j]].p].wave ? 1 : 0);
+        groupStreams.push_back(items[order[j]].t->tp[items[order[j]].p].nStream);
         size_t lds = 0;
         if (groupWave.back()) {
             // the largest slot file of the launch's items
@@ -2131,16 +2201,16 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     for (int q = 0; q < nItems; q++) {
         const Item &it = items[order[q]];
         const TreePart &tp = it.t->tp[it.p];
-        if (!tp.wave) continue;
+        if (!tp.wave || !pruneWaveVectorP(it.t->precision)) continue;
         const PruneArgs &pa = b->hArgs[q];
-        if (!hipOk(launchPgather(pa.PT, pa.wpre + 2 * pa.nPre, tp.nKidTot, tp.K, tp.PP, it.t->precision, s),
+        if (!hipOk(launchPgather(pa.PT, pa.wpre + 3 * pa.nPre, tp.nKidTot, tp.K, tp.PP, it.t->precision, s),
                    "launch P gather"))
             return 1;
     }
     for (size_t g = 0; g < launches.size(); g++) {
         const int *bi = b->dBlockItem + 2 * launches[g].first;
         const hipError_t le = groupWave[g] ? launchPruneWaveBatch(b->dArgs, bi, launches[g].second, groupK[g],
-                                                                 groupPrec[g], groupLds[g], s)
+                                                                 groupStreams[g], groupPrec[g], groupLds[g], s)
                               : pruneUsesMfma(groupDim[g], groupK[g], groupPrec[g])
                                   ? launchPruneAaBatch(b->dArgs, bi, launches[g].second, groupK[g], s)
                                   : launchPruneBatch(b->dArgs, bi, launches[g].second, groupDim[g], groupPrec[g],

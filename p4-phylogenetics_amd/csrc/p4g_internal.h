@@ -203,6 +203,7 @@ struct TreePart {
     // layout, else nPatPad).
     bool wave = false;
     int nWOps = 0, nWKids = 0, nPre = 0, nLds = 0, tipLds = 0, nKidTot = 0, pLds = 0;
+    int nStream = 1, nS0 = 0;
     void *PP = nullptr;            // [nNodes][K][16] of float or double
     size_t waveLds = 0, waveLdsNoP = 0;
     size_t sclPerSlot = 0;
@@ -296,10 +297,12 @@ struct PruneArgs {
     // wave-per-category DNA kernel
     const WaveOp *wops;
     const WaveKid *wkids;
-    const int *wpre;         // [nPre][2]: partials slot, LDS slot
+    const int *wpre;         // [nPre][3]: partials slot, LDS slot, stream
     int nWOps, nPre, nLds;   // nLds: LDS slots per wave
     int tipLds;              // 1: the tile's tip codes are staged in LDS
     int nTax;
+    int nStream, nS0;        // wave sets per category; stream 0 runs ops [0, nS0),
+                             // stream 1 [nS0, nWOps - 1), the root is last
     // P^T of every child in program order, [nKidTot][K][16] (k_pgather)
     const void *PP;
     int nKidTot, pLds;
@@ -325,14 +328,15 @@ hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, in
 // [slot][cat][pattern]; 64 patterns per workgroup of 64*K threads.
 bool pruneUsesWave(int dim, int K, int precision);
 bool pruneWaveVectorP(int precision);   // the wave kernel reads the k_pgather P stream
+bool pruneWaveStreams(int precision);   // the wave kernel splits the tree into two streams
 size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int precision);
 int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision);
 hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s);
 // P^T of the program's children in program order: PP[k][j] = PT[kidP[j]][k]
 hipError_t launchPgather(const void *PT, const int *kidP, int nKidTot, int K, void *PP, int precision,
                          hipStream_t s);
-hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, int precision, size_t lds,
-                                hipStream_t s);
+hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, int nStream, int precision,
+                                size_t lds, hipStream_t s);
 
 // Host numerics (p4g_host.hip).
 int discreteGamma(double *freqK, double *rK, double alfa, double beta, int K, int median);

@@ -665,17 +665,19 @@ static int g_tuneG[2] = {4, 4};      // [precision]
 static int g_tuneLds[2] = {1, 1};
 static int g_wave[2] = {1, 1};       // 1: 4-state parts use k_pruneWave
 static int g_waveVecP[2] = {0, 0};   // 1: its vector-prefetched P stream
+static int g_waveStreams[2] = {0, 0};  // 1: two streams of category waves (scalar P)
 
 // G = 0 selects the wave-per-category kernel for 4-state parts with its
 // vector-prefetched P stream, G = -1 the same kernel with P by scalar
-// loads; G = 1 or 4 the grouped kernel with that many ops per group (other
-// values: 4).
+// loads, G = -2 that with the tree split into two streams of waves; G = 1
+// or 4 the grouped kernel with that many ops per group (other values: 4).
 void setPruneTuning(int precision, int G, int lds)
 {
     if (precision < 0 || precision > 1) return;
-    if (G == 0 || G == -1) {
+    if (G == 0 || G == -1 || G == -2) {
         g_wave[precision] = 1;
         g_waveVecP[precision] = G == 0 ? 1 : 0;
+        g_waveStreams[precision] = G == -2 ? 1 : 0;
         return;
     }
     g_wave[precision] = 0;
@@ -1079,7 +1081,11 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
     __shared__ double rootL[WAVE_MAX_K][64];
     __shared__ int rootE[WAVE_MAX_K][64];
     const int K = a.K;
-    const int cat = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // wave wv = (stream, category); a stream is one set of category waves
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int nS = a.nStream;
+    const int stream = wv / K;
+    const int cat = wv - stream * K;
     const int lane = threadIdx.x & 63;
     const int p = tile * 64 + lane;
     const size_t nPP = (size_t)a.nPatPad;
@@ -1095,7 +1101,8 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
     typedef __attribute__((address_space(1))) const int gint;
     typedef __attribute__((address_space(1))) const unsigned char gu8;
     lu8 *sTip = (lu8 *)(smemW);
-    lT *slotT = (lT *)(smemW + tipBytes) + ((size_t)cat * nL * 64 + lane) * 4;   // slot s: slotT[s * 256 + 0..3]
+    lT *slotT = (lT *)(smemW + tipBytes) + ((size_t)wv * nL * 64 + lane) * 4;   // slot s: slotT[s * 256 + 0..3]
+    lT *slot0T = (lT *)(smemW + tipBytes) + ((size_t)cat * nL * 64 + lane) * 4;  // stream 0's file, same category
     auto ldS = [&](int sl) {
         V4 x;
         x.x = slotT[sl * 256];
@@ -1110,7 +1117,8 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
         slotT[sl * 256 + 2] = x.z;
         slotT[sl * 256 + 3] = x.w;
     };
-    lint *sexp = (lint *)(smemW + tipBytes + sizeof(V4) * (size_t)K * nL * 64) + (size_t)cat * nL * 64 + lane;
+    lint *sexp = (lint *)(smemW + tipBytes + sizeof(V4) * (size_t)nS * K * nL * 64) + (size_t)wv * nL * 64 + lane;
+    lint *sexp0 = (lint *)(smemW + tipBytes + sizeof(V4) * (size_t)nS * K * nL * 64) + (size_t)cat * nL * 64 + lane;
     gu8 *gTips = (gu8 *)(a.tips);
     gint *gScl = (gint *)(a.scl);
     if (a.tipLds) {
@@ -1136,25 +1144,16 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
     const size_t catOff = (size_t)cat * nPP + p;   // element of slot s: s*K*nPP + catOff
     const size_t slotStride = (size_t)K * nPP;
 
-    // clean children read by the program: loads issued in pairs before
-    // their LDS stores
+    // clean children read by this stream's ops, loaded into its slot file
     const int *pre = a.wpre;
-    for (int i = 0; i < a.nPre; i += 2) {
-        const bool two = i + 1 < a.nPre;
-        const size_t o0 = (size_t)__builtin_amdgcn_readfirstlane(pre[2 * i]) * slotStride + catOff;
-        const size_t o1 = two ? (size_t)__builtin_amdgcn_readfirstlane(pre[2 * i + 2]) * slotStride + catOff : o0;
-        const V4 x0 = ldG(o0);
-        const V4 x1 = ldG(o1);
-        const int e0 = SCALE ? gScl[o0] : 0;
-        const int e1 = SCALE ? gScl[o1] : 0;
-        const int l0 = __builtin_amdgcn_readfirstlane(pre[2 * i + 1]);
-        stS(l0, x0);
-        if (SCALE) sexp[l0 * 64] = e0;
-        if (two) {
-            const int l1 = __builtin_amdgcn_readfirstlane(pre[2 * i + 3]);
-            stS(l1, x1);
-            if (SCALE) sexp[l1 * 64] = e1;
-        }
+    for (int i = 0; i < a.nPre; i++) {
+        if (__builtin_amdgcn_readfirstlane(pre[3 * i + 2]) != stream) continue;
+        const size_t o = (size_t)__builtin_amdgcn_readfirstlane(pre[3 * i]) * slotStride + catOff;
+        const V4 x = ldG(o);
+        const int ex = SCALE ? gScl[o] : 0;
+        const int l = __builtin_amdgcn_readfirstlane(pre[3 * i + 1]);
+        stS(l, x);
+        if (SCALE) sexp[l * 64] = ex;
     }
 
     // Program records and P^T.  PV = 1: records and the program-order P^T
@@ -1185,7 +1184,20 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
             Pn1[j] = nKT > 1 ? PPg[((size_t)K + cat) * 16 + j] : T(0);
         }
     }
-    for (int i = 0; i < nW; i++) {
+    // stream 0 runs ops [0, nS0), stream 1 [nS0, nW - 1) (one stream: all
+    // but the root); then stream 0 runs the root after a barrier
+    const int sBeg = stream == 0 ? 0 : a.nS0;
+    const int sEnd = (nS == 1 || stream == 1) ? nW - 1 : a.nS0;
+    for (int it = sBeg; it <= sEnd; it++) {
+        const bool rootIter = it == sEnd;
+        if (rootIter) {
+            __syncthreads();   // stream 1's root children are in stream 0's slot file
+            if (stream != 0) {
+                __syncthreads();   // joins the category combine of the root
+                break;
+            }
+        }
+        const int i = rootIter ? nW - 1 : it;
         int nk, opDst, opLds, opFlags, kidStart;
         int4 r1, r2;
         T P0[16], P1[16];
@@ -1328,6 +1340,13 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
             if (ld >= 0) {
                 stS(ld, r);
                 if (SCALE) sexp[ld * 64] = e;
+            } else if (ld <= -2) {   // a root child of stream 1: junction slot in stream 0's file
+                const int j = -ld - 2;
+                slot0T[j * 256] = r.x;
+                slot0T[j * 256 + 1] = r.y;
+                slot0T[j * 256 + 2] = r.z;
+                slot0T[j * 256 + 3] = r.w;
+                if (SCALE) sexp0[j * 64] = e;
             }
             continue;
         }
@@ -1353,7 +1372,7 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
         rootL[cat][lane] = Lk;
         rootE[cat][lane] = e;
         __syncthreads();
-        if (cat == 0) {
+        if (wv == 0) {
             int mn = rootE[0][lane];
             for (int k = 1; k < K; k++) mn = min(mn, rootE[k][lane]);
             double L = 0.0;
@@ -1361,7 +1380,7 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
             siteLogLike<SCALE>(a, L, mn, p, K, pi, contrib, bad);
         }
     }
-    if (cat == 0) finishSiteSumWave(a, contrib, bad, tile, nTiles);
+    if (wv == 0) finishSiteSumWave(a, contrib, bad, tile, nTiles);
 }
 
 template <typename T, bool SCALE, bool PV>
@@ -1376,7 +1395,7 @@ __global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneWaveBatch(const PruneA
     const int item = blockItem[2 * blockIdx.x];
     const int local = blockItem[2 * blockIdx.x + 1];
     const PruneArgs a = args[item];
-    pruneWaveBody<T, SCALE, true>(a, local, a.nPatPad / 64);
+    pruneWaveBody<T, SCALE, false>(a, local, a.nPatPad / 64);
 }
 
 template <typename T>
@@ -1405,6 +1424,7 @@ hipError_t launchPgather(const void *PT, const int *kidP, int nKidTot, int K, vo
 }
 
 bool pruneWaveVectorP(int precision) { return g_waveVecP[precision & 1] != 0; }
+bool pruneWaveStreams(int precision) { return g_waveStreams[precision & 1] != 0 && !g_waveVecP[precision & 1]; }
 
 bool pruneUsesWave(int dim, int K, int precision)
 {
@@ -1435,8 +1455,8 @@ int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision)
 hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s)
 {
     // lds <= WAVE_LDS_BUDGET, within the default per-workgroup limit
-    const dim3 block(64 * a.K);
-    const bool pv = pruneWaveVectorP(precision);
+    const dim3 block(64 * a.K * a.nStream);
+    const bool pv = pruneWaveVectorP(precision) && a.nStream == 1;
     if (precision == FP32) {
         if (pv)
             hipLaunchKernelGGL((k_pruneWave<float, true, true>), dim3(nBlocks), block, lds, s, a);
@@ -1451,10 +1471,10 @@ hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_
     return hipGetLastError();
 }
 
-hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, int precision, size_t lds,
-                                hipStream_t s)
+hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, int nStream, int precision,
+                                size_t lds, hipStream_t s)
 {
-    const dim3 block(64 * K);
+    const dim3 block(64 * K * nStream);
     if (precision == FP32)
         hipLaunchKernelGGL((k_pruneWaveBatch<float, true>), dim3(nBlocks), block, lds, s, d, bi);
     else
