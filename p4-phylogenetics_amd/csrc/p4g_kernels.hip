@@ -1007,25 +1007,35 @@ template <> struct Vec4T<double> { typedef double4 type; };
 
 constexpr int WAVE_LDS_BUDGET = 48 * 1024;   // dynamic LDS per workgroup
 
-// leafVec for 4 states with P^T in registers: the lane's code selects
-// columns by masks, never by an index into the register array.  Same
-// values and summation order as leafVec.
+// leafVec for 4 states with P^T in registers.  A single-state code (the
+// common case) picks its row with a 2-bit select tree; gaps, '?' and
+// partial ambiguities are fixed up on a divergent path that is skipped
+// when no lane of the wave holds one.  Same values and summation order as
+// leafVec.
 template <typename T>
 __device__ __forceinline__ void leafVec4(const T *Pl, unsigned code, const uint64_t *amb, T *v)
 {
-    unsigned m;
-    if (code < 4u)
-        m = 1u << code;
-    else if (code == TIP_ALL)
-        m = 0u;
-    else
-        m = (unsigned)amb[code - 4] & 15u;
+    const bool b0 = code & 1u, b1 = code & 2u;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-        T x = T(0);
+        const T lo = b0 ? Pl[4 + s] : Pl[s];
+        const T hi = b0 ? Pl[12 + s] : Pl[8 + s];
+        v[s] = b1 ? hi : lo;
+    }
+    if (code >= 4u) {
+        if (code == TIP_ALL) {
 #pragma unroll
-        for (int j = 0; j < 4; j++) x += ((m >> j) & 1u) ? Pl[j * 4 + s] : T(0);
-        v[s] = (code == TIP_ALL) ? T(1) : x;
+            for (int s = 0; s < 4; s++) v[s] = T(1);
+        } else {
+            const unsigned m = (unsigned)amb[code - 4] & 15u;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                T x = T(0);
+#pragma unroll
+                for (int j = 0; j < 4; j++) x += ((m >> j) & 1u) ? Pl[j * 4 + s] : T(0);
+                v[s] = x;
+            }
+        }
     }
 }
 
@@ -1238,7 +1248,8 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
         }
         T acc[4];
         int e = 0;
-        for (int c = 0; c < nk; c++) {
+        // one child: its contribution v, folded into acc (first child sets it)
+        auto child = [&](int c) {
             int kind, idx;
             T Pl[16];
             if (PV) {
@@ -1315,6 +1326,13 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
 #pragma unroll
                 for (int s = 0; s < 4; s++) acc[s] *= v[s];
             }
+        };
+        if (nk == 2) {
+            // binary op (the common case): child indices known at compile time
+            child(0);
+            child(1);
+        } else {
+            for (int c = 0; c < nk; c++) child(c);
         }
         if (SCALE) {
             const T m = fmax(fmax(acc[0], acc[1]), fmax(acc[2], acc[3]));
