@@ -92,8 +92,8 @@ def main():
                     help="also run a plain device copy of known size (for PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the engine has no multi-GPU path (site patterns would be "
+              "sharded across GPUs with one all-reduce of the part sums); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from p4gpu import pf, synth
