@@ -309,6 +309,8 @@ static bool allocTreePart(Tree *t, int pNum)
         devFree(tp.blockSums);
         devFree(tp.blockBad);
         tp.alloc = false;
+        tp.progValid = false;
+        tp.layoutGen++;
         std::fill(tp.dirty.begin(), tp.dirty.end(), 1);
         tp.everComputed = false;
     }
@@ -694,18 +696,77 @@ static bool collectOps(Tree *t, int pNum, bool full, std::vector<OpRec> &ops, st
 }
 
 static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids);
+static bool buildGroupedProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids,
+                                int *nOpsOut, int *nKidsOut);
+
+// FNV-1a over everything the staged program depends on.
+static uint64_t programKey(const Tree *t, int pNum, bool full)
+{
+    const TreePart &tp = t->tp[pNum];
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](int64_t v) {
+        for (int b = 0; b < 8; b++) {
+            h ^= (uint64_t)((v >> (8 * b)) & 0xff);
+            h *= 1099511628211ull;
+        }
+    };
+    mix(full);
+    mix(tp.everComputed);
+    mix(tp.wave);
+    mix(t->precision);
+    mix(pruneWaveStreams(t->precision));
+    mix(pruneWaveVectorP(t->precision));
+    mix(pruneGroupWidthFor(tp.dim, t->precision));
+    mix(tp.layoutGen);
+    mix(t->root);
+    mix(t->nNodes);
+    for (int j = 0; j < t->nNodes; j++) mix(t->postOrder[j]);
+    for (int n = 0; n < t->nNodes; n++) {
+        const Node *nd = t->nodes[n];
+        mix(((int64_t)nd->parent << 32) ^ (int64_t)(uint32_t)nd->leftChild);
+        mix(((int64_t)nd->sibling << 32) ^ (int64_t)(uint32_t)nd->seqNum);
+        mix(nd->isLeaf);
+        mix(tp.dirty[n]);
+        mix(tp.pValid[n]);
+    }
+    return h;
+}
 
 static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsOut)
 {
     TreePart &tp = t->tp[pNum];
+    const uint64_t key = programKey(t, pNum, full);
+    if (tp.progValid && tp.progKey == key) {
+        // the staging area still holds this program
+        *nOpsOut = tp.progNOps;
+        *nKidsOut = tp.progNKids;
+        return true;
+    }
+    tp.progValid = false;
     std::vector<OpRec> ops;
     std::vector<KidRec> kids;
     if (!collectOps(t, pNum, full, ops, kids)) return false;
+    bool ok;
     if (tp.wave) {
         *nOpsOut = (int)ops.size();
         *nKidsOut = (int)kids.size();
-        return buildWaveProgram(t, pNum, ops, kids);
+        ok = buildWaveProgram(t, pNum, ops, kids);
+    } else {
+        ok = buildGroupedProgram(t, pNum, ops, kids, nOpsOut, nKidsOut);
     }
+    if (ok) {
+        tp.progValid = true;
+        tp.progKey = key;
+        tp.progNOps = *nOpsOut;
+        tp.progNKids = *nKidsOut;
+    }
+    return ok;
+}
+
+static bool buildGroupedProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids,
+                                int *nOpsOut, int *nKidsOut)
+{
+    TreePart &tp = t->tp[pNum];
     // Group ops by height above the nodes already computed: ops of equal
     // height are independent, so binary ones are packed G to a group.
     const int G = pruneGroupWidthFor(t->model->parts[pNum]->dim, t->precision);

@@ -204,6 +204,12 @@ struct TreePart {
     bool wave = false;
     int nWOps = 0, nWKids = 0, nPre = 0, nLds = 0, tipLds = 0, nKidTot = 0, pLds = 0;
     int nStream = 1, nS0 = 0;
+    // The staged program is reused while everything it depends on is
+    // unchanged (tree structure, postorder, marks, kernel choice, layout).
+    bool progValid = false;
+    uint64_t progKey = 0;
+    int progNOps = 0, progNKids = 0;
+    unsigned layoutGen = 0;
     void *PP = nullptr;            // [nNodes][K][16] of float or double
     size_t waveLds = 0, waveLdsNoP = 0;
     size_t sclPerSlot = 0;
