@@ -890,7 +890,6 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
                 }
     }
     const int tipLds = data->nTax <= WAVE_TIP_LDS_MAX_TAXA ? 1 : 0;
-    const int maxSlots = pruneWaveMaxSlots(tp.K, tipLds, data->nTax, t->precision);
     std::vector<int> pos(nOps);
     for (int q = 0; q < nOps; q++) pos[order[q]] = q;
     // Streams: with two, the subtrees under the root are split between two
@@ -1074,9 +1073,7 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     tp.nKidTot = nKidTot;
     tp.nStream = nStream;
     tp.nS0 = (int)lists[0].size();
-    tp.waveLdsNoP = pruneWaveLdsBytes(tp.K * nStream, nLds, tipLds, data->nTax, t->precision);
-    tp.pLds = 0;
-    tp.waveLds = tp.waveLdsNoP;
+    tp.waveLds = pruneWaveLdsBytes(tp.K * nStream, nLds, tipLds, data->nTax, t->precision);
     tp.progWords = (int)words;
     tp.nGroups = 0;
     tp.nKids = (int)kids.size();
@@ -1130,7 +1127,6 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
         a.nTax = dp->nTax;
         a.PP = tp.PP;
         a.nKidTot = tp.nKidTot;
-        a.pLds = tp.pLds;
     }
 }
 
@@ -2240,7 +2236,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         size_t lds = 0;
         if (groupWave.back()) {
             // the largest slot file of the launch's items
-            for (int q = j; q < jj; q++) lds = std::max(lds, items[order[q]].t->tp[items[order[q]].p].waveLdsNoP);
+            for (int q = j; q < jj; q++) lds = std::max(lds, items[order[q]].t->tp[items[order[q]].p].waveLds);
             groupLds.push_back(lds);
         } else {
             // P in LDS for the whole launch only if every item's P rows fit

@@ -202,7 +202,7 @@ struct TreePart {
     // wave, tips in LDS; per-slot exponent count (K*nPatPad for the wave
     // layout, else nPatPad).
     bool wave = false;
-    int nWOps = 0, nWKids = 0, nPre = 0, nLds = 0, tipLds = 0, nKidTot = 0, pLds = 0;
+    int nWOps = 0, nWKids = 0, nPre = 0, nLds = 0, tipLds = 0, nKidTot = 0;
     int nStream = 1, nS0 = 0;
     // The staged program is reused while everything it depends on is
     // unchanged (tree structure, postorder, marks, kernel choice, layout).
@@ -211,7 +211,7 @@ struct TreePart {
     int progNOps = 0, progNKids = 0;
     unsigned layoutGen = 0;
     void *PP = nullptr;            // [nNodes][K][16] of float or double
-    size_t waveLds = 0, waveLdsNoP = 0;
+    size_t waveLds = 0;
     size_t sclPerSlot = 0;
     // Host-side flags.
     std::vector<char> dirty;       // per node
@@ -311,7 +311,7 @@ struct PruneArgs {
                              // stream 1 [nS0, nWOps - 1), the root is last
     // P^T of every child in program order, [nKidTot][K][16] (k_pgather)
     const void *PP;
-    int nKidTot, pLds;
+    int nKidTot;
 };
 
 // Kernel launchers (p4g_kernels.hip).
