@@ -96,3 +96,25 @@ def test_prior_only_run_recovers_branch_length_prior():
     nBr = len(tree.nodes) - 1
     # Exponential(10) per branch: mean tree length nBr / 10
     assert abs(tl.mean() / (nBr / 10.0) - 1.0) < 0.1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("G", [-2, 0, 4])
+def test_incremental_equals_full_for_every_kernel_variant(G):
+    """Dirty-path evaluations (preloaded clean children, junction slots of
+    the two-stream variant, the program-order P stream, the grouped kernel)
+    equal a full recomputation, generation after generation."""
+    tree, _ = synth.make_workload("het64", seed=31, nSites=1200, precision="fp32")
+    try:
+        pf.setPruneTuning(True, G, 1)
+        m = Mcmc(tree, nChains=2, seed=9, verbose=False)
+        for g in range(60):
+            m.run(1, verbose=False)
+            for ch in m.chains:
+                full = ch.curTree.dupe()
+                full.precision = "fp32"
+                v = full.calcLogLike(verbose=0)
+                assert abs(v - ch.curTree.logLike) <= 1e-7 * abs(v), (G, g, v, ch.curTree.logLike)
+    finally:
+        pf.setPruneTuning(True, -1, 1)   # the library default, then the tuned choice
+        pf._applyTuning()
