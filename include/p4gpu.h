@@ -232,6 +232,15 @@ void p4g_setKernelTiming(int on);
  * stagePInLds.  Partials already computed are re-laid out (recomputed) at
  * the next evaluation when the choice changes the layout. */
 void p4g_setPruneTuning(int fp32, int opsPerGroup, int stagePInLds);
+/* How P matrices of 4-state parts are computed (no counterpart in the
+ * reference, which always goes through the eigensystem, Pf/eig.c:163-223):
+ * direct = 1 (the default) evaluates exp(Q t) per branch and category from
+ * the (pi, R) of its eigen slot by a scaled Taylor series of expm1, with no
+ * eigensystem; direct = 0 uses the eigensystem (computed when a slot
+ * changes) and P = I + V diag(expm1(lam t)) V^-1.  Other state counts always
+ * use the eigensystem.  Takes effect at the next flush of pending P
+ * matrices; eigensystems skipped meanwhile are recomputed when needed. */
+void p4g_setPmatRoute(int direct);
 double p4g_lastKernelMicroseconds(const p4g_tree *t);
 
 /* ---- learned proposal network (no counterpart in the reference, whose

@@ -378,6 +378,8 @@ static bool allocTreePart(Tree *t, int pNum)
             return false;
     }
     tp.eigPending.assign(tp.nEig, 0);
+    tp.eigSnap.assign(tp.nEig, 0);
+    tp.eigStale = false;
     tp.pPending.assign(t->nNodes, 0);
     if (!hipOk(hipMemsetAsync(tp.counter, 0, sizeof(unsigned), stream()), "zero counter")) return false;
     tp.dirty.assign(t->nNodes, 1);
@@ -452,6 +454,7 @@ static void markEig(Tree *t, int pNum, int slot)
     std::memcpy(dst, mp->comps[c].val, sizeof(double) * S);
     std::memcpy(dst + S, mp->rMatrices[r].bigR.data(), sizeof(double) * S * S);
     tp.eigPending[slot] = 1;
+    tp.eigSnap[slot] = 1;
 }
 
 // Snapshot the effective branch lengths of node n (Pf/p4_node.c:321-346)
@@ -468,15 +471,26 @@ static void markP(Tree *t, int pNum, const Node *nd)
 }
 
 // Pack the pending eigensystem slots and P items of one part into the
-// staging arena (no copies, no launches).
+// staging arena (no copies, no launches).  When P comes straight from the
+// (pi, R) snapshots no eigensystem is computed; the first flush on the
+// eigen route after that recomputes every slot that holds a snapshot.
 static void stageParams(Tree *t, int pNum)
 {
     TreePart &tp = t->tp[pNum];
     const int K = t->model->parts[pNum]->nCat;
+    const bool direct = pmatDirect(t->model->parts[pNum]->dim);
+    if (!direct && tp.eigStale) {
+        for (int i = 0; i < tp.nEig; i++)
+            if (tp.eigSnap[i]) tp.eigPending[i] = 1;
+        tp.eigStale = false;
+    }
     int nE = 0;
     for (int i = 0; i < tp.nEig; i++)
         if (tp.eigPending[i]) {
-            tp.hEigList[nE++] = i;
+            if (direct)
+                tp.eigStale = true;
+            else
+                tp.hEigList[nE++] = i;
             tp.eigPending[i] = 0;
         }
     int nI = 0;
@@ -522,7 +536,12 @@ static bool launchParams(Tree *t, int pNum)
         PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
                    reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
                    reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
-        if (!hipOk(launchPmat(a, s), "launch P kernel")) return false;
+        if (pmatDirect(S)) {
+            if (!hipOk(launchPexp(a, reinterpret_cast<const double *>(tp.dArena), s), "launch P kernel"))
+                return false;
+        } else if (!hipOk(launchPmat(a, s), "launch P kernel")) {
+            return false;
+        }
     }
     tp.pendE = tp.pendI = 0;
     return true;
@@ -1229,6 +1248,11 @@ void p4g_setKernelTiming(int on) { g_timing = on != 0; }
 void p4g_setPruneTuning(int fp32, int opsPerGroup, int stagePInLds)
 {
     setPruneTuning(fp32 ? FP32 : FP64, opsPerGroup, stagePInLds);
+}
+
+void p4g_setPmatRoute(int direct)
+{
+    setPmatRoute(direct);
 }
 
 double p4g_lastKernelMicroseconds(const p4g_tree *t)
@@ -1989,6 +2013,11 @@ int p4g_copyModelPrams(p4g_tree *a, p4g_tree *b)
                 !hipOk(hipMemcpyAsync(TB.eigVi, TA.eigVi, ne * dim * dim * 8, hipMemcpyDeviceToDevice, s), "copy eig") ||
                 !hipOk(hipMemcpyAsync(TB.eigLam, TA.eigLam, ne * dim * 8, hipMemcpyDeviceToDevice, s), "copy eig"))
                 return 1;
+            // the (pi, R) snapshots go with the eigensystems: P of tree b
+            // may be computed from them directly
+            std::memcpy(TB.hEigIn, TA.hEigIn, ne * (dim + dim * dim) * sizeof(double));
+            TB.eigSnap = TA.eigSnap;
+            TB.eigStale = TA.eigStale;
         }
         for (int g = 0; g < A->nGdasrvs; g++)
             if (A->gdasrvs[g]->free && A->gdasrvs[g] != B->gdasrvs[g]) {

@@ -232,6 +232,9 @@ struct TreePart {
     double *hRootPi = nullptr;     // [MAX_DIM]                       (arena)
     int *hProg = nullptr;          // groups, ops, kids               (arena)
     std::vector<char> eigPending;  // per eigen slot
+    std::vector<char> eigSnap;     // per eigen slot: hEigIn holds a snapshot
+    bool eigStale = false;         // snapshots taken while P came straight from
+                                   // (pi, R), so eigV/eigVi/eigLam lag them
     double *hT = nullptr;          // [nNodes][K] effective branch lengths
     int *hEs = nullptr;            // [nNodes] eigen slot of each node
     size_t packBytes = 0;
@@ -317,6 +320,11 @@ struct PruneArgs {
 // Kernel launchers (p4g_kernels.hip).
 hipError_t launchEig(const EigArgs &a, hipStream_t s);
 hipError_t launchPmat(const PmatArgs &a, hipStream_t s);
+// P = exp(Q t) per (branch, category) straight from the (pi, R) snapshots
+// eigIn (4-state parts only); pmatDirect(dim) says whether that route is on.
+hipError_t launchPexp(const PmatArgs &a, const double *eigIn, hipStream_t s);
+bool pmatDirect(int dim);
+void setPmatRoute(int direct);
 hipError_t launchPrune(const PruneArgs &a, int precision, int nBlocks, hipStream_t s);
 hipError_t launchPruneBatch(const PruneArgs *dArgs, const int *dBlockItem, int nBlocks,
                             int dim, int precision, size_t lds, hipStream_t s);

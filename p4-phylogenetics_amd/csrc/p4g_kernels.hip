@@ -242,6 +242,125 @@ hipError_t launchPmat(const PmatArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
+// ------------------------------------------- P by direct exponentiation
+// 4-state parts skip the eigensystem: one thread per (branch, category)
+// builds Q from the snapshot of (pi, R) (the normalisation of Pf/util.c:
+// 44-115, as in k_eig) and evaluates E = expm1(Q t) in registers:
+// B = Q t / 2^s with ||B||_inf <= 1/4, the Taylor series of expm1(B) to
+// degree 12 by Horner (first omitted term below 2.5e-18 ||B||), then s
+// doublings E <- 2E + E E, the expm1 form of squaring (I + E)^2 = I + 2E + E^2,
+// so that short branches keep their small off-diagonal entries to full
+// relative precision.  P = I + E is stored transposed, as k_pmat stores it.
+constexpr int PEXP_DEGREE = 12;
+
+template <int S>
+__device__ __forceinline__ void matMul(const double (&X)[S][S], const double (&Y)[S][S], double (&Z)[S][S])
+{
+#pragma unroll
+    for (int i = 0; i < S; i++)
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            double acc = X[i][0] * Y[0][j];
+#pragma unroll
+            for (int m = 1; m < S; m++) acc = fma(X[i][m], Y[m][j], acc);
+            Z[i][j] = acc;
+        }
+}
+
+template <int S>
+__global__ __launch_bounds__(64) void k_pexp(PmatArgs a, const double *in)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= a.nItems * a.K) return;
+    const int it = gid / a.K, k = gid % a.K;
+    const int node = a.nodeList[it];
+    const double t = a.t[(size_t)it * a.K + k];
+    const double *pi = in + (size_t)a.eigSlot[it] * (S + S * S);
+    const double *R = pi + S;
+    double B[S][S];
+    double sumODE = 0.0;
+#pragma unroll
+    for (int r = 0; r < S; r++) {
+        double row = 0.0;
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+            const double q = (r == c) ? 0.0 : R[r * S + c] * pi[c];
+            B[r][c] = q;
+            row += q;
+        }
+        B[r][r] = -row;
+        sumODE += pi[r] * row;
+    }
+    // B = Q t / sumODE, then its infinity norm picks the number of doublings
+    const double f = t / sumODE;
+    double nrm = 0.0;
+#pragma unroll
+    for (int r = 0; r < S; r++) {
+        double row = 0.0;
+#pragma unroll
+        for (int c = 0; c < S; c++) {
+            B[r][c] *= f;
+            row += fabs(B[r][c]);
+        }
+        nrm = fmax(nrm, row);
+    }
+    int sq = 0;
+    if (nrm > 0.25) {
+        (void)frexp(nrm * 4.0, &sq);  // nrm * 4 <= 2^sq
+    }
+#pragma unroll
+    for (int r = 0; r < S; r++)
+#pragma unroll
+        for (int c = 0; c < S; c++) B[r][c] = ldexp(B[r][c], -sq);
+    // Z = I + B/2 + B^2/3! + ... + B^(d-1)/d!  (Horner), E = B Z
+    double Z[S][S], W[S][S];
+#pragma unroll
+    for (int r = 0; r < S; r++)
+#pragma unroll
+        for (int c = 0; c < S; c++) Z[r][c] = (r == c ? 1.0 : 0.0) + B[r][c] * (1.0 / PEXP_DEGREE);
+#pragma unroll
+    for (int j = PEXP_DEGREE - 1; j >= 2; j--) {
+        matMul<S>(B, Z, W);
+        const double inv = 1.0 / j;
+#pragma unroll
+        for (int r = 0; r < S; r++)
+#pragma unroll
+            for (int c = 0; c < S; c++) Z[r][c] = (r == c ? 1.0 : 0.0) + W[r][c] * inv;
+    }
+    matMul<S>(B, Z, W);  // W = E
+    for (int i = 0; i < sq; i++) {
+        matMul<S>(W, W, Z);
+#pragma unroll
+        for (int r = 0; r < S; r++)
+#pragma unroll
+            for (int c = 0; c < S; c++) W[r][c] = fma(2.0, W[r][c], Z[r][c]);
+    }
+    const size_t o = ((size_t)node * a.K + k) * S * S;
+#pragma unroll
+    for (int j = 0; j < S; j++)
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            const double pij = (i == j ? 1.0 : 0.0) + W[i][j];
+            a.PT64[o + j * S + i] = pij;
+            a.PT32[o + j * S + i] = (float)pij;
+        }
+}
+
+static int g_pexpDirect = 1;
+
+void setPmatRoute(int direct) { g_pexpDirect = direct ? 1 : 0; }
+
+bool pmatDirect(int dim) { return g_pexpDirect && dim == 4; }
+
+hipError_t launchPexp(const PmatArgs &a, const double *eigIn, hipStream_t s)
+{
+    const int total = a.nItems * a.K;
+    if (total == 0) return hipSuccess;
+    if (a.dim != 4) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pexp<4>, dim3((total + 63) / 64), dim3(64), 0, s, a, eigIn);
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------- pruning
 constexpr int PRUNE_THREADS = 256;
 constexpr float SCALE_THRESH = 5.9604645e-08f;  // 2^-24

@@ -76,6 +76,7 @@ def _declare():
     _sig("p4g_deviceAvailable", I)
     _sig("p4g_setKernelTiming", None, I)
     _sig("p4g_setPruneTuning", None, I, I, I)
+    _sig("p4g_setPmatRoute", None, I)
     _sig("p4g_lastKernelMicroseconds", D, V)
     _sig("p4g_newData", V, I, I)
     _sig("p4g_freeData", None, V)
@@ -187,6 +188,12 @@ def setKernelTiming(on):
 
 def setPruneTuning(fp32, opsPerGroup, stagePInLds):
     lib().p4g_setPruneTuning(1 if fp32 else 0, int(opsPerGroup), 1 if stagePInLds else 0)
+
+
+def setPmatRoute(direct):
+    """4-state P matrices straight from exp(Q t) (True, the default) or from
+    the eigensystem (False)."""
+    lib().p4g_setPmatRoute(1 if direct else 0)
 
 
 def lastKernelMicroseconds(cTree):

@@ -254,3 +254,91 @@ def test_wave_kernel_variants_match_golden(prec, tol):
     finally:
         pf.setPruneTuning(fp32, -1, 1)   # the library default, then the tuned choice
         pf._applyTuning()
+
+
+def _bigq_normalised(pi, R):
+    S = len(pi)
+    Q = R * pi[None, :]
+    numpy.fill_diagonal(Q, 0.0)
+    numpy.fill_diagonal(Q, -Q.sum(1))
+    return Q / -(pi * numpy.diag(Q)).sum()
+
+
+@pytest.mark.parametrize("case", ["gtr16_500", "het64_800"])
+def test_p_routes_match_golden(case):
+    """4-state P matrices straight from exp(Q t) (the default) and from the
+    eigensystem both give the reference's P matrices and lnL; switching
+    from the direct route to the eigen route mid-life recomputes the
+    eigensystems skipped meanwhile."""
+    fx = load(case)
+    K, S = int(fx["nCat"]), int(fx["dim"])
+    try:
+        for route in (True, False, True):
+            pf.setPmatRoute(route)
+            t = tree_from_fixture(fx, "fp64")
+            assert rel(t.calcLogLike(verbose=0), float(fx["lnL"])) < FP64_TOL, route
+            for n in range(1, len(fx["P"])):
+                if n == int(fx["root"]):
+                    continue
+                got = pf.p4_getBigPDeck(t.nodes[n].cNode, 0, K, S)
+                assert numpy.abs(got - fx["P"][n]).max() < 2e-14, (route, n)
+        # direct route, then the eigen route on the same tree: a branch
+        # change now needs the eigensystems that the direct route skipped
+        pf.setPmatRoute(True)
+        t = tree_from_fixture(fx, "fp64")
+        t.calcLogLike(verbose=0)
+        pf.setPmatRoute(False)
+        nd = t.nodes[1]
+        pf.p4_setBrLen(nd.cNode, 0.2)
+        pf.p4_calculateBigPDecks(nd.cNode)
+        got = pf.p4_getBigPDeck(nd.cNode, 0, K, S)
+        c, r = int(fx["compNums"][1]), int(fx["rMatrixNums"][1])
+        Q = _bigq_normalised(numpy.asarray(fx["comps"][c], float), numpy.asarray(fx["bigR"][r], float))
+        g = int(fx["gdasrvNums"][1])
+        for k in range(K):
+            tk = 0.2 * float(fx["rates"][g][k]) * float(fx["relRate"]) / (1.0 - float(fx["pInvar"]))
+            want = scipy_expm(Q * tk)
+            assert numpy.abs(got[k] - want).max() < 1e-13, k
+    finally:
+        pf.setPmatRoute(True)
+
+
+def scipy_expm(A):
+    import scipy.linalg
+    return scipy.linalg.expm(A)
+
+
+def test_direct_p_extreme_branch_lengths():
+    """exp(Q t) by the scaled Taylor series against scipy's expm and the
+    eigen route, from 1e-9 to 200 substitutions per site; short branches
+    keep their off-diagonal entries to full relative precision."""
+    fx = load("gtr16_500")
+    K, S = int(fx["nCat"]), int(fx["dim"])
+    lens = [1e-9, 1e-5, 1e-3, 0.05, 0.7, 3.0, 20.0, 200.0]
+    pi = numpy.asarray(fx["comps"][0], float)
+    Q = _bigq_normalised(pi, numpy.asarray(fx["bigR"][0], float))
+    rates = numpy.asarray(fx["rates"][0], float) * float(fx["relRate"]) / (1.0 - float(fx["pInvar"]))
+    decks = {}
+    try:
+        for route in (True, False):
+            pf.setPmatRoute(route)
+            t = tree_from_fixture(fx, "fp64")
+            t.calcLogLike(verbose=0)
+            nodes = [n for n in range(1, len(fx["parent"])) if n != int(fx["root"])][:len(lens)]
+            for n, L in zip(nodes, lens):
+                pf.p4_setBrLen(t.nodes[n].cNode, L)
+                pf.p4_calculateBigPDecks(t.nodes[n].cNode)
+            decks[route] = [pf.p4_getBigPDeck(t.nodes[n].cNode, 0, K, S) for n in nodes]
+    finally:
+        pf.setPmatRoute(True)
+    off = ~numpy.eye(S, dtype=bool)
+    for i, L in enumerate(lens):
+        for k in range(K):
+            want = scipy_expm(Q * L * rates[k])
+            got = decks[True][i][k]
+            tol = 1e-14 if L * rates[k] < 5 else 1e-11
+            assert numpy.abs(got - want).max() < tol, (L, k)
+            assert numpy.abs(got - decks[False][i][k]).max() < max(tol, 1e-12), (L, k)
+            assert numpy.allclose(got.sum(1), 1.0, rtol=0, atol=1e-13)
+            if L * rates[k] < 1e-2:
+                assert (numpy.abs(got - want)[off] / want[off]).max() < 1e-12, (L, k)
