@@ -95,6 +95,17 @@ static bool hostAlloc(T **p, size_t n, const char *what)
     return true;
 }
 
+// Device address of pinned host memory: kernels write results there
+// directly, and the host reads them after the stream synchronises.
+template <typename T>
+static bool hostDevPtr(T *h, T **d, const char *what)
+{
+    void *q = nullptr;
+    if (!hipOk(hipHostGetDevicePointer(&q, h, 0), what)) return false;
+    *d = (T *)q;
+    return true;
+}
+
 template <typename T>
 static void hostFree(T *&p)
 {
@@ -135,7 +146,6 @@ static void freeTreePart(TreePart &t)
     devFree(t.blockSums);
     devFree(t.blockBad);
     devFree(t.counter);
-    devFree(t.result);
     devFree(t.patLikes);
     devFree(t.dArena);
     hostFree(t.hArena);
@@ -344,7 +354,7 @@ static bool allocTreePart(Tree *t, int pNum)
         !devAlloc(&tp.PT64, nPS, "hipMalloc P64") || !devAlloc(&tp.PT32, nPS, "hipMalloc P32") ||
         !devAlloc(&tp.blockSums, tp.maxBlocks, "hipMalloc blockSums") ||
         !devAlloc(&tp.blockBad, tp.maxBlocks, "hipMalloc blockBad") ||
-        !devAlloc(&tp.counter, 1, "hipMalloc counter") || !devAlloc(&tp.result, 2, "hipMalloc result") ||
+        !devAlloc(&tp.counter, 1, "hipMalloc counter") ||
         !devAlloc(&tp.patLikes, tp.nPatPad, "hipMalloc patLikes"))
         return false;
     if (!tp.cl && !hipOk(hipMalloc(&tp.cl, tp.clBytes * tp.nSlots), "hipMalloc partials")) return false;
@@ -366,8 +376,16 @@ static bool allocTreePart(Tree *t, int pNum)
             !devAlloc(&tp.dArena, tp.arenaBytes, "hipMalloc staging arena") ||
             !hostAlloc(&tp.hT, (size_t)t->nNodes * K, "hipHostMalloc branch lengths") ||
             !hostAlloc(&tp.hEs, t->nNodes, "hipHostMalloc eigen slots") ||
-            !hostAlloc(&tp.hResult, 2, "hipHostMalloc result"))
+            !hostAlloc(&tp.hResult, 2, "hipHostMalloc result") ||
+            !hostDevPtr(tp.hResult, &tp.dResultHost, "device address of result"))
             return false;
+        char *dah = nullptr;
+        if (!hostDevPtr(tp.hArena, &dah, "device address of staging arena")) return false;
+        tp.dArenaHost = dah;
+        tp.eigSnap.assign(tp.nEig, 0);
+        tp.eigStale = false;
+        tp.progOnDevice = false;
+        tp.rootPiOnDevice = false;
         tp.hEigIn = reinterpret_cast<double *>(tp.hArena);
         tp.hEigList = reinterpret_cast<int *>(tp.hArena + tp.offEigList);
         tp.hPack = tp.hArena + tp.offPack;
@@ -378,8 +396,6 @@ static bool allocTreePart(Tree *t, int pNum)
             return false;
     }
     tp.eigPending.assign(tp.nEig, 0);
-    tp.eigSnap.assign(tp.nEig, 0);
-    tp.eigStale = false;
     tp.pPending.assign(t->nNodes, 0);
     if (!hipOk(hipMemsetAsync(tp.counter, 0, sizeof(unsigned), stream()), "zero counter")) return false;
     tp.dirty.assign(t->nNodes, 1);
@@ -532,12 +548,16 @@ static bool launchParams(Tree *t, int pNum)
     }
     if (tp.pendI) {
         const int nI = tp.pendI;
-        const char *d = tp.dArena + tp.offPack;
+        // the direct route reads branch lengths, nodes, slots and (pi, R)
+        // from the pinned arena itself; the eigen route from its upload
+        const bool direct = pmatDirect(S);
+        const char *base = direct ? tp.dArenaHost : tp.dArena;
+        const char *d = base + tp.offPack;
         PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
                    reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
                    reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
-        if (pmatDirect(S)) {
-            if (!hipOk(launchPexp(a, reinterpret_cast<const double *>(tp.dArena), s), "launch P kernel"))
+        if (direct) {
+            if (!hipOk(launchPexp(a, reinterpret_cast<const double *>(base), s), "launch P kernel"))
                 return false;
         } else if (!hipOk(launchPmat(a, s), "launch P kernel")) {
             return false;
@@ -552,7 +572,8 @@ static bool flushParams(Tree *t, int pNum)
     stageParams(t, pNum);
     TreePart &tp = t->tp[pNum];
     if (!tp.pendE && !tp.pendI) return true;
-    return uploadArena(tp, tp.offPi) && launchParams(t, pNum);
+    if (!pmatDirect(t->model->parts[pNum]->dim) && !uploadArena(tp, tp.offPi)) return false;
+    return launchParams(t, pNum);
 }
 
 // Flush every part and wait (for inspection and copy entry points).
@@ -762,6 +783,7 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
         return true;
     }
     tp.progValid = false;
+    tp.progOnDevice = false;
     std::vector<OpRec> ops;
     std::vector<KidRec> kids;
     if (!collectOps(t, pNum, full, ops, kids)) return false;
@@ -1131,7 +1153,7 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     a.blockSums = tp.blockSums;
     a.blockBad = tp.blockBad;
     a.counter = tp.counter;
-    a.result = tp.result;
+    a.result = tp.dResultHost;
     a.patLikes = siteLikes ? tp.patLikes : nullptr;
     if (tp.wave) {
         a.wops = reinterpret_cast<const WaveOp *>(prog);
@@ -1163,11 +1185,30 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps)
         return false;
     }
     std::memcpy(tp.hRootPi, mp->comps[rc].val, sizeof(double) * mp->dim);
-    // pending eigensystems and P, the program and the root comp: one upload,
-    // then the parameter kernels; the pruning launch follows on the stream
+    // what the device copy of the arena lacks -- the eigen route's pending
+    // eigensystems and P, changed root frequencies, a rebuilt program -- goes
+    // up in one copy of the covering range (none when nothing changed); then
+    // the parameter kernels; the pruning launch follows on the stream
     stageParams(t, pNum);
     (void)nKids;
-    return uploadArena(tp, tp.offProg + sizeof(int) * tp.progWords) && launchParams(t, pNum);
+    size_t lo = tp.arenaBytes, hi = 0;
+    auto need = [&](size_t a, size_t b) {
+        lo = std::min(lo, a);
+        hi = std::max(hi, b);
+    };
+    const size_t piBytes = sizeof(double) * mp->dim;
+    if (!pmatDirect(mp->dim) && (tp.pendE || tp.pendI)) need(0, tp.offPi);
+    if (!tp.rootPiOnDevice || std::memcmp(tp.rootPiUp, tp.hRootPi, piBytes) != 0) need(tp.offPi, tp.offPi + piBytes);
+    if (!tp.progOnDevice) need(tp.offProg, tp.offProg + sizeof(int) * tp.progWords);
+    if (hi > lo) {
+        if (!hipOk(hipMemcpyAsync(tp.dArena + lo, tp.hArena + lo, hi - lo, hipMemcpyHostToDevice, stream()),
+                   "stage inputs"))
+            return false;
+        std::memcpy(tp.rootPiUp, tp.hRootPi, piBytes);
+        tp.rootPiOnDevice = true;
+        tp.progOnDevice = true;
+    }
+    return launchParams(t, pNum);
 }
 
 static void finishPart(Tree *t, int pNum, bool siteLikes, double lnL, bool bad)
@@ -1202,9 +1243,6 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
                                                                     : launchPrune(a, t->precision, nBlocks, s);
     if (!hipOk(le, "launch pruning kernel")) return false;
     if (g_timing) (void)hipEventRecord(tp.ev1, s);
-    if (!hipOk(hipMemcpyAsync(tp.hResult, tp.result, 2 * sizeof(double), hipMemcpyDeviceToHost, s),
-               "copy result"))
-        return false;
     std::vector<double> pl;
     if (siteLikes) {
         pl.resize(tp.nPat);
@@ -2229,9 +2267,12 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         return ((mp->dim * 2 + prec) * 16 + kc) * 32 + wk;
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
+    double *dRes = nullptr;
+    if (!hostDevPtr(b->hRes, &dRes, "device address of batch results")) return 1;
     for (int j = 0; j < nItems; j++) {
         const Item &it = items[order[j]];
         fillPruneArgs(it.t, it.p, it.nOps, b->hArgs[j], false);
+        b->hArgs[j].result = dRes + 2 * j;  // lnL and bad flag straight into pinned memory
     }
     hipStream_t s = stream();
     if (!hipOk(hipMemcpyAsync(b->dArgs, b->hArgs, sizeof(PruneArgs) * nItems, hipMemcpyHostToDevice, s),
@@ -2302,13 +2343,6 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
                                   : launchPruneBatch(b->dArgs, bi, launches[g].second, groupDim[g], groupPrec[g],
                                                      groupLds[g], s);
         if (!hipOk(le, "launch batched pruning kernel")) return 1;
-    }
-    for (int q = 0; q < nItems; q++) {
-        const Item &it = items[order[q]];
-        if (!hipOk(hipMemcpyAsync(b->hRes + 2 * q, it.t->tp[it.p].result, 2 * sizeof(double),
-                                  hipMemcpyDeviceToHost, s),
-                   "copy batch results"))
-            return 1;
     }
     if (!hipOk(hipStreamSynchronize(s), "batched pruning kernel")) return 1;
     for (size_t i = 0; i < b->trees.size(); i++) lnL[i] = 0.0;

@@ -193,7 +193,6 @@ struct TreePart {
     double *blockSums = nullptr;
     int *blockBad = nullptr;
     unsigned *counter = nullptr;
-    double *result = nullptr;      // [2] lnL, bad flag
     double *patLikes = nullptr;    // [nPatPad]
     int maxBlocks = 0;
     // Staged program (device pointers into the arena).
@@ -241,6 +240,12 @@ struct TreePart {
     std::vector<char> pPending;    // per node
     int pendE = 0, pendI = 0;      // staged counts for the next launch
     double *hResult = nullptr;     // pinned [2]
+    double *dResultHost = nullptr; // device address of hResult: the kernel writes lnL there
+    const char *dArenaHost = nullptr;  // device address of hArena: the 4-state P kernel
+                                       // reads its staged inputs there, no upload
+    bool progOnDevice = false;     // dArena holds the staged program
+    bool rootPiOnDevice = false;   // dArena holds rootPiUp as the root frequencies
+    double rootPiUp[MAX_DIM] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double lastKernelUs = -1.0;
 };
