@@ -241,6 +241,13 @@ void p4g_setPruneTuning(int fp32, int opsPerGroup, int stagePInLds);
  * use the eigensystem.  Takes effect at the next flush of pending P
  * matrices; eigensystems skipped meanwhile are recomputed when needed. */
 void p4g_setPmatRoute(int direct);
+/* Layout of the 20-state FP32 (MFMA) pruning launch (no counterpart in the
+ * reference): with categoryWaves = 1 (the default) a program of at least
+ * minOps node ops (default 32: full evaluations, not short dirty paths)
+ * runs one wavefront per (16-pattern tile, gamma category), with one
+ * barrier per op; otherwise one wavefront per tile holds every category.
+ * Partials are the same either way. */
+void p4g_setAaTuning(int categoryWaves, int minOps);
 double p4g_lastKernelMicroseconds(const p4g_tree *t);
 
 /* ---- learned proposal network (no counterpart in the reference, whose

@@ -345,7 +345,8 @@ static bool allocTreePart(Tree *t, int pNum)
     tp.nEig = mp->nComps * mp->nRMatrices;
     const size_t nPS = (size_t)t->nNodes * K * S * S;
     // blocks of the launch: the largest count any pruning kernel may use
-    tp.maxBlocks = tp.nPatPad / std::min({pruneBlockPatternsFor(S, K, t->precision), pruneBlockPatterns(K), 64});
+    // enough block sums for any kernel choice (the smallest tile: 16 patterns)
+    tp.maxBlocks = tp.nPatPad / std::min({pruneBlockPatternsFor(S, K, t->precision, 0), pruneBlockPatterns(K), 16});
     const size_t elt = t->precision == FP32 ? sizeof(float) : sizeof(double);
     tp.clBytes = (size_t)tp.nPatPad * K * S * elt;
     if (!devAlloc(&tp.eigV, (size_t)tp.nEig * S * S, "hipMalloc eigV") ||
@@ -1231,7 +1232,7 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
     PruneArgs a{};
     fillPruneArgs(t, pNum, nOps, a, siteLikes);
     hipStream_t s = stream();
-    const int nBlocks = tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision);
+    const int nBlocks = tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision, nOps);
     if (tp.wave && pruneWaveVectorP(t->precision)) {
         const int *kidP = a.wpre + 3 * tp.nPre;
         if (!hipOk(launchPgather(a.PT, kidP, tp.nKidTot, tp.K, tp.PP, t->precision, s), "launch P gather"))
@@ -1291,6 +1292,11 @@ void p4g_setPruneTuning(int fp32, int opsPerGroup, int stagePInLds)
 void p4g_setPmatRoute(int direct)
 {
     setPmatRoute(direct);
+}
+
+void p4g_setAaTuning(int categoryWaves, int minOps)
+{
+    setAaTuning(categoryWaves, minOps);
 }
 
 double p4g_lastKernelMicroseconds(const p4g_tree *t)
@@ -2233,7 +2239,8 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     auto itemBlocks = [](const Item &it) {
         const ModelPart *mp = it.t->model->parts[it.p];
         const TreePart &tp = it.t->tp[it.p];
-        return tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(mp->dim, mp->nCat, it.t->precision);
+        return tp.wave ? tp.nPatPad / 64
+                       : tp.nPatPad / pruneBlockPatternsFor(mp->dim, mp->nCat, it.t->precision, it.nOps);
     };
     for (auto &it : items) nBlocksTot += itemBlocks(it);
     if (nItems > b->capItems) {
@@ -2260,11 +2267,15 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     auto key = [&](int i) {
         const ModelPart *mp = items[i].t->model->parts[items[i].p];
         const int prec = items[i].t->precision;
-        const int kc = pruneUsesMfma(mp->dim, mp->nCat, prec) ? (mp->nCat <= 1 ? 1 : mp->nCat <= 2 ? 2 : mp->nCat <= 4 ? 4 : 8) : 0;
+        // MFMA kernels: one launch per (category count, kernel choice) (the
+        // category-split kernel's workgroup is 64 * K threads)
+        const int kc = pruneUsesMfma(mp->dim, mp->nCat, prec)
+                           ? 2 * mp->nCat + (pruneAaCategoryWaves(items[i].nOps) ? 1 : 0)
+                           : 0;
         // the wave kernel's workgroup size is 64 * K: one launch per K
         const TreePart &tq = items[i].t->tp[items[i].p];
         const int wk = tq.wave ? 16 * tq.nStream + mp->nCat : 0;
-        return ((mp->dim * 2 + prec) * 16 + kc) * 32 + wk;
+        return ((mp->dim * 2 + prec) * 64 + kc) * 32 + wk;
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
     double *dRes = nullptr;
@@ -2282,7 +2293,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     std::vector<std::pair<int, int>> launches;  // (first block, count) per group
     std::vector<int> groupDim, groupPrec;
     std::vector<size_t> groupLds;
-    std::vector<int> groupK, groupWave, groupStreams;
+    std::vector<int> groupK, groupWave, groupStreams, groupAaCat;
     while (j < nItems) {
         const int k0 = key(order[j]);
         const int first = blk;
@@ -2301,6 +2312,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         groupDim.push_back(items[order[j]].t->model->parts[items[order[j]].p]->dim);
         groupPrec.push_back(items[order[j]].t->precision);
         groupK.push_back(items[order[j]].t->model->parts[items[order[j]].p]->nCat);
+        groupAaCat.push_back(pruneAaCategoryWaves(items[order[j]].nOps) ? 1 : 0);
         groupWave.push_back(items[order[j]].t->tp[items[order[j]].p].wave ? 1 : 0);
         groupStreams.push_back(items[order[j]].t->tp[items[order[j]].p].nStream);
         size_t lds = 0;
@@ -2339,7 +2351,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         const hipError_t le = groupWave[g] ? launchPruneWaveBatch(b->dArgs, bi, launches[g].second, groupK[g],
                                                                  groupStreams[g], groupPrec[g], groupLds[g], s)
                               : pruneUsesMfma(groupDim[g], groupK[g], groupPrec[g])
-                                  ? launchPruneAaBatch(b->dArgs, bi, launches[g].second, groupK[g], s)
+                                  ? launchPruneAaBatch(b->dArgs, bi, launches[g].second, groupK[g], groupAaCat[g] != 0, s)
                                   : launchPruneBatch(b->dArgs, bi, launches[g].second, groupDim[g], groupPrec[g],
                                                      groupLds[g], s);
         if (!hipOk(le, "launch batched pruning kernel")) return 1;

@@ -330,6 +330,12 @@ hipError_t launchPmat(const PmatArgs &a, hipStream_t s);
 hipError_t launchPexp(const PmatArgs &a, const double *eigIn, hipStream_t s);
 bool pmatDirect(int dim);
 void setPmatRoute(int direct);
+// 20-state FP32 pruning: one wavefront per (tile, category) for programs
+// of at least minOps ops when catWaves is on (the default), else one per
+// tile with every category.
+constexpr int AA_CAT_MIN_OPS = 32;
+void setAaTuning(int catWaves, int minOps);
+bool pruneAaCategoryWaves(int nOps);
 hipError_t launchPrune(const PruneArgs &a, int precision, int nBlocks, hipStream_t s);
 hipError_t launchPruneBatch(const PruneArgs *dArgs, const int *dBlockItem, int nBlocks,
                             int dim, int precision, size_t lds, hipStream_t s);
@@ -339,9 +345,9 @@ void setPruneTuning(int precision, int G, int lds);
 size_t pruneLdsBytes(int nKids, int K, int dim, int precision);  // 0 if P stays in global
 // 20-state FP32 parts use the MFMA kernel and a tile-blocked partial layout
 bool pruneUsesMfma(int dim, int K, int precision);
-int pruneBlockPatternsFor(int dim, int K, int precision);
+int pruneBlockPatternsFor(int dim, int K, int precision, int nOps);
 hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s);
-hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, hipStream_t s);
+hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, hipStream_t s);
 // 4-state parts with at most WAVE_MAX_K categories use the wave-per-category
 // kernel: partials [slot][cat][pattern][state], FP32 exponents
 // [slot][cat][pattern]; 64 patterns per workgroup of 64*K threads.

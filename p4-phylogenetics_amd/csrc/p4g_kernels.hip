@@ -1065,6 +1065,172 @@ __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneAa(PruneArgs a)
     pruneAaBody<KC>(a, blockIdx.x, gridDim.x);
 }
 
+__device__ __forceinline__ void finishSiteSumWave(const PruneArgs &a, double contrib, int bad, int blk, int nBlk);
+
+// Category-split variant: one wavefront per (16-pattern tile, category),
+// the K category waves of a tile in one workgroup, so a launch has K times
+// the wavefronts and each walks the program with a K-th of the work.  The
+// scale exponent stays per (node, pattern): every wave posts its
+// per-pattern maximum in LDS (double-buffered, one barrier per op) and all
+// take the maximum over categories, so partials and exponents are those of
+// k_pruneAa bit for bit; only the FP64 root sum over categories is added
+// in another order.
+__device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int nTiles)
+{
+    __shared__ float mx[2][WAVE_MAX_K][AA_TILE];
+    __shared__ double rootL[WAVE_MAX_K][AA_TILE];
+    const int tid = threadIdx.x;
+    const int k = __builtin_amdgcn_readfirstlane(tid >> 6);   // category
+    const int lane = tid & 63;
+    const int q = lane >> 4, pl = lane & 15;
+    const int K = a.K;
+    const int p = tile * AA_TILE + pl;
+    const size_t nPatPad = (size_t)a.nPatPad;
+    const size_t tileFloats = (size_t)K * AA_S * AA_TILE;
+    const size_t slotFloats = nPatPad * K * AA_S;
+    const float *PT = (const float *)a.PT;
+    float *cl = (float *)a.cl;
+    const f4v zero = {0.f, 0.f, 0.f, 0.f};
+    const f4v ones = {1.f, 1.f, 1.f, 1.f};
+
+    double contrib = 0.0;
+    int bad = 0;
+    int par = 0;
+    for (int g = 0; g < a.nGroups; ++g) {
+        const GroupRec gr = a.groups[g];
+        for (int o = 0; o < gr.nOps; ++o) {
+            const OpRec op = a.ops[gr.opStart + o];
+            f4v pr0 = zero, pr1 = zero;
+            int e = 0;
+            for (int c = 0; c < op.nKids; ++c) {
+                const KidRec kd = a.kids[op.kidStart + c];
+                const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
+                f4v v0, v1;
+                bool skip = false;
+                if (kd.kind == 0) {
+                    const unsigned code = a.tips[(size_t)kd.idx * nPatPad + p];
+                    skip = (code == TIP_ALL) && c != 0;
+                    if (code < (unsigned)AA_S) {
+                        v0 = *reinterpret_cast<const f4v *>(Pk + code * AA_S + 4 * q);
+                        v1 = q == 0 ? *reinterpret_cast<const f4v *>(Pk + code * AA_S + 16) : zero;
+                    } else if (code == TIP_ALL) {
+                        v0 = ones;
+                        v1 = q == 0 ? ones : zero;
+                    } else {
+                        const uint64_t m = a.ambMask[code - AA_S];
+                        v0 = zero;
+                        v1 = zero;
+                        for (int j = 0; j < AA_S; j++)
+                            if ((m >> j) & 1ull) {
+                                v0 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 4 * q);
+                                if (q == 0) v1 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 16);
+                            }
+                    }
+                } else {
+                    const float *X = cl + (size_t)kd.idx * slotFloats + (size_t)tile * tileFloats;
+                    f4v acc0 = zero, acc1 = zero;
+#pragma unroll
+                    for (int t = 0; t < 5; t++) {
+                        const float b = X[(k * 5 + t) * 64 + lane];
+                        const float a0 = Pk[(4 * t + q) * AA_S + pl];
+                        const float a1 = pl < 4 ? Pk[(4 * t + q) * AA_S + 16 + pl] : 0.f;
+                        acc0 = mfma4(a0, b, acc0);
+                        acc1 = mfma4(a1, b, acc1);
+                    }
+                    v0 = acc0;
+                    v1 = acc1;
+                    e += a.scl[(size_t)kd.idx * nPatPad + p];
+                }
+                if (c == 0) {
+                    pr0 = v0;
+                    pr1 = v1;
+                } else if (!skip) {
+                    pr0 *= v0;
+                    pr1 *= v1;
+                }
+            }
+            // rescale: max over this wave's states, then over the categories
+            float m = 0.f;
+            for (int r = 0; r < 4; r++) m = fmaxf(m, fmaxf(pr0[r], pr1[r]));
+            m = fmaxf(m, __shfl_xor(m, 16));
+            m = fmaxf(m, __shfl_xor(m, 32));
+            if (q == 0) mx[par][k][pl] = m;
+            __syncthreads();
+            m = 0.f;
+            for (int kk = 0; kk < K; kk++) m = fmaxf(m, mx[par][kk][pl]);
+            par ^= 1;   // the next op posts into the other buffer: no second barrier
+            if (m < SCALE_THRESH && m > 0.f) {
+                int ex;
+                frexpf(m, &ex);
+                const int E = -ex;
+                for (int r = 0; r < 4; r++) {
+                    pr0[r] = ldexpf(pr0[r], E);
+                    pr1[r] = ldexpf(pr1[r], E);
+                }
+                e += E;
+            }
+            if (!(op.flags & OP_ROOT)) {
+                float *D = cl + (size_t)op.dst * slotFloats + (size_t)tile * tileFloats;
+                for (int r = 0; r < 4; r++) D[((k * 5 + q) * 4 + r) * AA_TILE + pl] = pr0[r];
+                if (q == 0)
+                    for (int r = 0; r < 4; r++) D[((k * 5 + 4) * 4 + r) * AA_TILE + pl] = pr1[r];
+                if (k == 0 && q == 0) a.scl[(size_t)op.dst * nPatPad + p] = e;
+                continue;
+            }
+            // root: this category's sum_s pi_s cl_s across the four lanes of
+            // the pattern, then wave 0 adds the categories in order
+            const double *pi = a.rootPi;
+            uint64_t msk = ~0ull;
+            if (a.rootLeafRow >= 0) {
+                const unsigned code = a.tips[(size_t)a.rootLeafRow * nPatPad + p];
+                msk = code < (unsigned)AA_S ? (1ull << code) : code == TIP_ALL ? ~0ull : a.ambMask[code - AA_S];
+            }
+            double Lk = 0.0;
+            for (int r = 0; r < 4; r++) {
+                const int s0 = 4 * q + r;
+                if ((msk >> s0) & 1ull) Lk += pi[s0] * (double)pr0[r];
+                if (q == 0 && ((msk >> (16 + r)) & 1ull)) Lk += pi[16 + r] * (double)pr1[r];
+            }
+            Lk += __shfl_xor(Lk, 16);
+            Lk += __shfl_xor(Lk, 32);
+            if (q == 0) rootL[k][pl] = Lk;
+            __syncthreads();
+            if (k == 0 && q == 0) {
+                double L = 0.0;
+                for (int kk = 0; kk < K; kk++) L += rootL[kk][pl];
+                siteLogLike<true>(a, L, e, p, K, pi, contrib, bad);
+            }
+        }
+    }
+    if (k == 0) finishSiteSumWave(a, contrib, bad, tile, nTiles);
+}
+
+__global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneAaCat(PruneArgs a)
+{
+    pruneAaCatBody(a, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneAaCatBatch(const PruneArgs *args, const int *blockItem)
+{
+    const int item = blockItem[2 * blockIdx.x];
+    const int local = blockItem[2 * blockIdx.x + 1];
+    const PruneArgs a = args[item];
+    pruneAaCatBody(a, local, a.nPatPad / AA_TILE);
+}
+
+// Category waves pay one barrier per op; short programs (dirty paths)
+// run faster on the tile kernel, long ones (full evaluations) on these.
+static int g_aaCat = 1;
+static int g_aaCatMinOps = AA_CAT_MIN_OPS;
+
+void setAaTuning(int catWaves, int minOps)
+{
+    g_aaCat = catWaves ? 1 : 0;
+    g_aaCatMinOps = minOps < 0 ? 0 : minOps;
+}
+
+bool pruneAaCategoryWaves(int nOps) { return g_aaCat != 0 && nOps >= g_aaCatMinOps; }
+
 template <int KC>
 __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneAaBatch(const PruneArgs *args, const int *blockItem)
 {
@@ -1076,15 +1242,23 @@ __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneAaBatch(const PruneArgs 
 
 bool pruneUsesMfma(int dim, int K, int precision) { return dim == AA_S && precision == FP32 && K <= 8; }
 
-int pruneBlockPatternsFor(int dim, int K, int precision)
+int pruneBlockPatternsFor(int dim, int K, int precision, int nOps)
 {
-    return pruneUsesMfma(dim, K, precision) ? AA_TILE * PRUNE_THREADS / 64 : pruneBlockPatterns(K);
+    if (pruneUsesMfma(dim, K, precision))
+        return pruneAaCategoryWaves(nOps) ? AA_TILE : AA_TILE * PRUNE_THREADS / 64;
+    return pruneBlockPatterns(K);
 }
 
 static int aaKC(int K) { return K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : 8; }
 
 hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s)
 {
+    if (pruneAaCategoryWaves(a.nOps)) {
+        if (a.K < 1 || a.K > WAVE_MAX_K || nBlocks * AA_TILE > a.nPatPad) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_pruneAaCat, dim3(nBlocks), dim3(64 * a.K), 0, s, a);
+        return hipGetLastError();
+    }
+    if (nBlocks * AA_TILE * (PRUNE_THREADS / 64) > a.nPatPad) return hipErrorInvalidValue;
     switch (aaKC(a.K)) {
     case 1: hipLaunchKernelGGL(k_pruneAa<1>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a); break;
     case 2: hipLaunchKernelGGL(k_pruneAa<2>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, a); break;
@@ -1094,8 +1268,14 @@ hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, hipStream_t s)
+hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, hipStream_t s)
 {
+    if (catWaves) {
+        // every item of the launch has exactly K categories (the batch groups by K)
+        if (K < 1 || K > WAVE_MAX_K) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_pruneAaCatBatch, dim3(nBlocks), dim3(64 * K), 0, s, d, bi);
+        return hipGetLastError();
+    }
     switch (aaKC(K)) {
     case 1: hipLaunchKernelGGL(k_pruneAaBatch<1>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi); break;
     case 2: hipLaunchKernelGGL(k_pruneAaBatch<2>, dim3(nBlocks), dim3(PRUNE_THREADS), 0, s, d, bi); break;

@@ -77,6 +77,7 @@ def _declare():
     _sig("p4g_setKernelTiming", None, I)
     _sig("p4g_setPruneTuning", None, I, I, I)
     _sig("p4g_setPmatRoute", None, I)
+    _sig("p4g_setAaTuning", None, I, I)
     _sig("p4g_lastKernelMicroseconds", D, V)
     _sig("p4g_newData", V, I, I)
     _sig("p4g_freeData", None, V)
@@ -194,6 +195,16 @@ def setPmatRoute(direct):
     """4-state P matrices straight from exp(Q t) (True, the default) or from
     the eigensystem (False)."""
     lib().p4g_setPmatRoute(1 if direct else 0)
+
+
+AA_CAT_MIN_OPS = 32
+
+
+def setAaTuning(categoryWaves, minOps=AA_CAT_MIN_OPS):
+    """20-state FP32 pruning with one wavefront per (tile, category) for
+    programs of at least minOps node ops (categoryWaves True, the default),
+    else one wavefront per tile."""
+    lib().p4g_setAaTuning(1 if categoryWaves else 0, int(minOps))
 
 
 def lastKernelMicroseconds(cTree):

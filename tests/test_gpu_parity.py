@@ -342,3 +342,49 @@ def test_direct_p_extreme_branch_lengths():
             assert numpy.allclose(got.sum(1), 1.0, rtol=0, atol=1e-13)
             if L * rates[k] < 1e-2:
                 assert (numpy.abs(got - want)[off] / want[off]).max() < 1e-12, (L, k)
+
+
+@pytest.mark.parametrize("nCat", [4, 3])
+def test_protein_category_waves_match_tile_waves(nCat):
+    """The 20-state MFMA kernel with one wavefront per (tile, category)
+    stores the same partials and exponents, bit for bit, as the one with a
+    wavefront per tile holding every category; lnL agrees to 1e-12 (only
+    the FP64 root sum over categories is added in another order) and with
+    the FP64 path to the FP32 tolerance.  nCat 3: 192-thread workgroups."""
+    fx = dict(load("prot128_150"))
+    fx["nCat"] = numpy.array(nCat)
+    K, S, nP = nCat, int(fx["dim"]), fx["patterns"].shape[1]
+    want = tree_from_fixture(fx, "fp64").calcLogLike(verbose=0)
+    if nCat == 4:
+        assert rel(want, float(fx["lnL"])) < FP64_TOL
+    res = {}
+    try:
+        for cw in (False, True):
+            pf.setAaTuning(cw, 0)
+            t = tree_from_fixture(fx, "fp32")
+            v = t.calcLogLike(verbose=0)
+            assert rel(v, want) < FP32_TOL, (cw, v, want)
+            cls = [pf.p4_getCondLikes(n.cNode, 0, K, S, nP) for n in t.nodes if not (n.isLeaf or n is t.root)]
+            res[cw] = (v, cls)
+    finally:
+        pf.setAaTuning(True)
+    assert rel(res[True][0], res[False][0]) < 1e-12
+    for a, b in zip(res[True][1], res[False][1]):
+        assert numpy.array_equal(a, b)
+
+
+def test_batch_mixes_protein_kernels():
+    """One batched call where one tree's protein parts run long programs
+    (category waves) and another's short dirty paths (tile kernel): every
+    lnL equals the tree's full evaluation."""
+    trees = [synth.make_workload("multi48", seed=s, nSites=1500, precision="fp32")[0] for s in (6, 7)]
+    want = [t.calcLogLike(verbose=0) for t in trees]
+    for n in trees[0].iterInternals():
+        pf.p4_setConditionalLikelihoodsOfInternalNode(n.cNode)
+    n = next(iter(trees[1].iterLeavesNoRoot())).parent
+    while n is not None:
+        pf.p4_setConditionalLikelihoodsOfInternalNode(n.cNode)
+        n = n.parent
+    got = pf.Batch([t.cTree for t in trees]).logLikes()
+    for g, w in zip(got, want):
+        assert rel(g, w) < 1e-12, (g, w)
