@@ -232,14 +232,16 @@ void p4g_setKernelTiming(int on);
  * stagePInLds.  Partials already computed are re-laid out (recomputed) at
  * the next evaluation when the choice changes the layout. */
 void p4g_setPruneTuning(int fp32, int opsPerGroup, int stagePInLds);
-/* How P matrices of 4-state parts are computed (no counterpart in the
- * reference, which always goes through the eigensystem, Pf/eig.c:163-223):
- * direct = 1 (the default) evaluates exp(Q t) per branch and category from
- * the (pi, R) of its eigen slot by a scaled Taylor series of expm1, with no
- * eigensystem; direct = 0 uses the eigensystem (computed when a slot
- * changes) and P = I + V diag(expm1(lam t)) V^-1.  Other state counts always
- * use the eigensystem.  Takes effect at the next flush of pending P
- * matrices; eigensystems skipped meanwhile are recomputed when needed. */
+/* How P matrices are computed (no counterpart in the reference, which
+ * always goes through the eigensystem, Pf/eig.c:163-223): direct = 1 (the
+ * default) evaluates exp(Q t) per branch and category from the (pi, R) of
+ * its eigen slot by a scaled Taylor series of expm1, with no eigensystem,
+ * for 4-state parts always and for 20-state parts when at least 16 P
+ * matrices are pending (fewer reuse cached eigensystems); direct = 0 uses
+ * the eigensystem (computed when a slot changes) and
+ * P = I + V diag(expm1(lam t)) V^-1.  Other state counts always use the
+ * eigensystem.  Takes effect at the next flush of pending P matrices;
+ * eigensystems skipped meanwhile are recomputed when needed. */
 void p4g_setPmatRoute(int direct);
 /* Layout of the 20-state FP32 (MFMA) pruning launch (no counterpart in the
  * reference): with categoryWaves = 1 (the default) a program of at least

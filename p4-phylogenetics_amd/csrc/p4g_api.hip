@@ -495,7 +495,14 @@ static void stageParams(Tree *t, int pNum)
 {
     TreePart &tp = t->tp[pNum];
     const int K = t->model->parts[pNum]->nCat;
-    const bool direct = pmatDirect(t->model->parts[pNum]->dim);
+    const int dim = t->model->parts[pNum]->dim;
+    int nI = 0;
+    for (int n = 0; n < t->nNodes; n++) nI += tp.pPending[n] ? 1 : 0;
+    // 20-state P straight from exp(Qt) is one latency-bound wavefront per
+    // matrix: worth it for many matrices (a Q change, a full evaluation),
+    // not for the few of a branch move, which reuse cached eigensystems
+    const bool direct = pmatDirect(dim) && (dim == 4 || nI >= PEXP_WAVE_MIN_ITEMS);
+    tp.flushDirect = direct;
     if (!direct && tp.eigStale) {
         for (int i = 0; i < tp.nEig; i++)
             if (tp.eigSnap[i]) tp.eigPending[i] = 1;
@@ -510,8 +517,6 @@ static void stageParams(Tree *t, int pNum)
                 tp.hEigList[nE++] = i;
             tp.eigPending[i] = 0;
         }
-    int nI = 0;
-    for (int n = 0; n < t->nNodes; n++) nI += tp.pPending[n] ? 1 : 0;
     double *ht = reinterpret_cast<double *>(tp.hPack);
     int *hn = reinterpret_cast<int *>(ht + (size_t)nI * K);
     int *he = hn + nI;
@@ -551,7 +556,7 @@ static bool launchParams(Tree *t, int pNum)
         const int nI = tp.pendI;
         // the direct route reads branch lengths, nodes, slots and (pi, R)
         // from the pinned arena itself; the eigen route from its upload
-        const bool direct = pmatDirect(S);
+        const bool direct = tp.flushDirect;
         const char *base = direct ? tp.dArenaHost : tp.dArena;
         const char *d = base + tp.offPack;
         PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
@@ -573,7 +578,7 @@ static bool flushParams(Tree *t, int pNum)
     stageParams(t, pNum);
     TreePart &tp = t->tp[pNum];
     if (!tp.pendE && !tp.pendI) return true;
-    if (!pmatDirect(t->model->parts[pNum]->dim) && !uploadArena(tp, tp.offPi)) return false;
+    if (!tp.flushDirect && !uploadArena(tp, tp.offPi)) return false;
     return launchParams(t, pNum);
 }
 
@@ -1198,7 +1203,7 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps)
         hi = std::max(hi, b);
     };
     const size_t piBytes = sizeof(double) * mp->dim;
-    if (!pmatDirect(mp->dim) && (tp.pendE || tp.pendI)) need(0, tp.offPi);
+    if (!tp.flushDirect && (tp.pendE || tp.pendI)) need(0, tp.offPi);
     if (!tp.rootPiOnDevice || std::memcmp(tp.rootPiUp, tp.hRootPi, piBytes) != 0) need(tp.offPi, tp.offPi + piBytes);
     if (!tp.progOnDevice) need(tp.offProg, tp.offProg + sizeof(int) * tp.progWords);
     if (hi > lo) {

@@ -232,6 +232,7 @@ struct TreePart {
     int *hProg = nullptr;          // groups, ops, kids               (arena)
     std::vector<char> eigPending;  // per eigen slot
     std::vector<char> eigSnap;     // per eigen slot: hEigIn holds a snapshot
+    bool flushDirect = false;      // the staged P matrices come straight from exp(Qt)
     bool eigStale = false;         // snapshots taken while P came straight from
                                    // (pi, R), so eigV/eigVi/eigLam lag them
     double *hT = nullptr;          // [nNodes][K] effective branch lengths
@@ -329,6 +330,8 @@ hipError_t launchPmat(const PmatArgs &a, hipStream_t s);
 // eigIn (4-state parts only); pmatDirect(dim) says whether that route is on.
 hipError_t launchPexp(const PmatArgs &a, const double *eigIn, hipStream_t s);
 bool pmatDirect(int dim);
+// 20-state flushes with fewer pending P matrices use the eigen route.
+constexpr int PEXP_WAVE_MIN_ITEMS = 16;
 void setPmatRoute(int direct);
 // 20-state FP32 pruning: one wavefront per (tile, category) for programs
 // of at least minOps ops when catWaves is on (the default), else one per

@@ -346,18 +346,135 @@ __global__ __launch_bounds__(64) void k_pexp(PmatArgs a, const double *in)
         }
 }
 
+// The same series and doublings for larger state counts (20: proteins),
+// one wavefront per (branch, category): B, Z and W live in LDS and lane l
+// owns entries l, l + 64, ...; every product sums over m in the same order
+// as matMul.
+template <int S>
+__global__ __launch_bounds__(64) void k_pexpWave(PmatArgs a, const double *in)
+{
+    constexpr int NE = S * S;
+    constexpr int PER = (NE + 63) / 64;
+    __shared__ double B[NE], Z[NE], W[NE];
+    __shared__ double red[S];
+    __shared__ double sumODE;
+    const int lane = threadIdx.x;
+    const int it = blockIdx.x / a.K, k = blockIdx.x - (blockIdx.x / a.K) * a.K;
+    if (it >= a.nItems) return;   // the whole workgroup
+    const int node = a.nodeList[it];
+    const double t = a.t[(size_t)it * a.K + k];
+    const double *pi = in + (size_t)a.eigSlot[it] * (S + NE);
+    const double *R = pi + S;
+    for (int e = lane; e < NE; e += 64) {
+        const int r = e / S, c = e - (e / S) * S;
+        B[e] = (r == c) ? 0.0 : R[e] * pi[c];
+    }
+    __syncthreads();
+    if (lane < S) {
+        double row = 0.0;
+        for (int c = 0; c < S; c++) row += B[lane * S + c];
+        B[lane * S + lane] = -row;
+        red[lane] = pi[lane] * row;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double sOde = 0.0;
+        for (int r = 0; r < S; r++) sOde += red[r];
+        sumODE = sOde;
+    }
+    __syncthreads();
+    const double f = t / sumODE;
+    for (int e = lane; e < NE; e += 64) B[e] *= f;
+    __syncthreads();
+    if (lane < S) {
+        double row = 0.0;
+        for (int c = 0; c < S; c++) row += fabs(B[lane * S + c]);
+        red[lane] = row;
+    }
+    __syncthreads();
+    double nrm = 0.0;
+    for (int r = 0; r < S; r++) nrm = fmax(nrm, red[r]);
+    int sq = 0;
+    if (nrm > 0.25) {
+        (void)frexp(nrm * 4.0, &sq);
+    }
+    for (int e = lane; e < NE; e += 64) {
+        const int r = e / S, c = e - (e / S) * S;
+        const double b = ldexp(B[e], -sq);
+        B[e] = b;
+        Z[e] = (r == c ? 1.0 : 0.0) + b * (1.0 / PEXP_DEGREE);
+    }
+    __syncthreads();
+    // acc[i] = (X Y)[entry lane + 64 i]
+    auto product = [&](const double *X, const double *Y, double (&acc)[PER]) {
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int e = lane + 64 * i;
+            acc[i] = 0.0;
+            if (e < NE) {
+                const int r = e / S, c = e - (e / S) * S;
+                double v = X[r * S] * Y[c];
+                for (int m = 1; m < S; m++) v = fma(X[r * S + m], Y[m * S + c], v);
+                acc[i] = v;
+            }
+        }
+    };
+    double acc[PER];
+    for (int j = PEXP_DEGREE - 1; j >= 2; j--) {
+        product(B, Z, acc);
+        __syncthreads();
+        const double inv = 1.0 / j;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int e = lane + 64 * i;
+            if (e < NE) {
+                const int r = e / S, c = e - (e / S) * S;
+                Z[e] = (r == c ? 1.0 : 0.0) + acc[i] * inv;
+            }
+        }
+        __syncthreads();
+    }
+    product(B, Z, acc);  // E
+#pragma unroll
+    for (int i = 0; i < PER; i++)
+        if (lane + 64 * i < NE) W[lane + 64 * i] = acc[i];
+    __syncthreads();
+    for (int d = 0; d < sq; d++) {
+        product(W, W, acc);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int e = lane + 64 * i;
+            if (e < NE) W[e] = fma(2.0, W[e], acc[i]);
+        }
+        __syncthreads();
+    }
+    const size_t o = ((size_t)node * a.K + k) * NE;
+    for (int e = lane; e < NE; e += 64) {
+        const int i = e / S, j = e - (e / S) * S;
+        const double pij = (i == j ? 1.0 : 0.0) + W[e];
+        a.PT64[o + j * S + i] = pij;
+        a.PT32[o + j * S + i] = (float)pij;
+    }
+}
+
 static int g_pexpDirect = 1;
 
 void setPmatRoute(int direct) { g_pexpDirect = direct ? 1 : 0; }
 
-bool pmatDirect(int dim) { return g_pexpDirect && dim == 4; }
+bool pmatDirect(int dim) { return g_pexpDirect && (dim == 4 || dim == 20); }
 
 hipError_t launchPexp(const PmatArgs &a, const double *eigIn, hipStream_t s)
 {
     const int total = a.nItems * a.K;
     if (total == 0) return hipSuccess;
-    if (a.dim != 4) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_pexp<4>, dim3((total + 63) / 64), dim3(64), 0, s, a, eigIn);
+    if (a.dim == 4) {
+        hipLaunchKernelGGL(k_pexp<4>, dim3((total + 63) / 64), dim3(64), 0, s, a, eigIn);
+    } else if (a.dim == 20) {
+        hipLaunchKernelGGL(k_pexpWave<20>, dim3(total), dim3(64), 0, s, a, eigIn);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

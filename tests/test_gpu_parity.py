@@ -264,10 +264,10 @@ def _bigq_normalised(pi, R):
     return Q / -(pi * numpy.diag(Q)).sum()
 
 
-@pytest.mark.parametrize("case", ["gtr16_500", "het64_800"])
+@pytest.mark.parametrize("case", ["gtr16_500", "het64_800", "prot128_150"])
 def test_p_routes_match_golden(case):
-    """4-state P matrices straight from exp(Q t) (the default) and from the
-    eigensystem both give the reference's P matrices and lnL; switching
+    """4- and 20-state P matrices straight from exp(Q t) (the default) and
+    from the eigensystem both give the reference's P matrices and lnL; switching
     from the direct route to the eigen route mid-life recomputes the
     eigensystems skipped meanwhile."""
     fx = load(case)
@@ -282,23 +282,25 @@ def test_p_routes_match_golden(case):
                     continue
                 got = pf.p4_getBigPDeck(t.nodes[n].cNode, 0, K, S)
                 assert numpy.abs(got - fx["P"][n]).max() < 2e-14, (route, n)
-        # direct route, then the eigen route on the same tree: a branch
-        # change now needs the eigensystems that the direct route skipped
-        pf.setPmatRoute(True)
-        t = tree_from_fixture(fx, "fp64")
-        t.calcLogLike(verbose=0)
-        pf.setPmatRoute(False)
-        nd = t.nodes[1]
-        pf.p4_setBrLen(nd.cNode, 0.2)
-        pf.p4_calculateBigPDecks(nd.cNode)
-        got = pf.p4_getBigPDeck(nd.cNode, 0, K, S)
-        c, r = int(fx["compNums"][1]), int(fx["rMatrixNums"][1])
-        Q = _bigq_normalised(numpy.asarray(fx["comps"][c], float), numpy.asarray(fx["bigR"][r], float))
-        g = int(fx["gdasrvNums"][1])
-        for k in range(K):
-            tk = 0.2 * float(fx["rates"][g][k]) * float(fx["relRate"]) / (1.0 - float(fx["pInvar"]))
-            want = scipy_expm(Q * tk)
-            assert numpy.abs(got[k] - want).max() < 1e-13, k
+        # a full evaluation on the direct route, then a one-branch change on
+        # the eigen route (chosen, or taken by 20-state parts for a flush of
+        # few matrices): it needs the eigensystems that the direct route skipped
+        for second in (False, True):
+            pf.setPmatRoute(True)
+            t = tree_from_fixture(fx, "fp64")
+            t.calcLogLike(verbose=0)
+            pf.setPmatRoute(second)
+            nd = t.nodes[1]
+            pf.p4_setBrLen(nd.cNode, 0.2)
+            pf.p4_calculateBigPDecks(nd.cNode)
+            got = pf.p4_getBigPDeck(nd.cNode, 0, K, S)
+            c, r = int(fx["compNums"][1]), int(fx["rMatrixNums"][1])
+            Q = _bigq_normalised(numpy.asarray(fx["comps"][c], float), numpy.asarray(fx["bigR"][r], float))
+            g = int(fx["gdasrvNums"][1])
+            for k in range(K):
+                tk = 0.2 * float(fx["rates"][g][k]) * float(fx["relRate"]) / (1.0 - float(fx["pInvar"]))
+                want = scipy_expm(Q * tk)
+                assert numpy.abs(got[k] - want).max() < 1e-13, (second, k)
     finally:
         pf.setPmatRoute(True)
 
