@@ -533,31 +533,25 @@ static void stageParams(Tree *t, int pNum)
     tp.pendI = nI;
 }
 
-// One host-to-device copy of the arena prefix [0, bytes).
-static bool uploadArena(TreePart &tp, size_t bytes)
-{
-    return hipOk(hipMemcpyAsync(tp.dArena, tp.hArena, bytes, hipMemcpyHostToDevice, stream()), "stage inputs");
-}
 
-// Launch the staged eigensystems, then the staged P matrices (they read the
-// uploaded arena).
+// Launch the staged eigensystems, then the staged P matrices.  Both read
+// their inputs (slot list, (pi, R) snapshots, branch lengths, nodes, slots)
+// from the pinned arena itself, through its device address: no upload.
 static bool launchParams(Tree *t, int pNum)
 {
     TreePart &tp = t->tp[pNum];
     ModelPart *mp = t->model->parts[pNum];
     const int S = mp->dim, K = mp->nCat;
     hipStream_t s = stream();
+    const char *base = tp.dArenaHost;
     if (tp.pendE) {
-        EigArgs a{S, tp.pendE, reinterpret_cast<const int *>(tp.dArena + tp.offEigList),
-                  reinterpret_cast<const double *>(tp.dArena), tp.eigV, tp.eigVi, tp.eigLam};
+        EigArgs a{S, tp.pendE, reinterpret_cast<const int *>(base + tp.offEigList),
+                  reinterpret_cast<const double *>(base), tp.eigV, tp.eigVi, tp.eigLam};
         if (!hipOk(launchEig(a, s), "launch eigensystem kernel")) return false;
     }
     if (tp.pendI) {
         const int nI = tp.pendI;
-        // the direct route reads branch lengths, nodes, slots and (pi, R)
-        // from the pinned arena itself; the eigen route from its upload
         const bool direct = tp.flushDirect;
-        const char *base = direct ? tp.dArenaHost : tp.dArena;
         const char *d = base + tp.offPack;
         PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
                    reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
@@ -578,7 +572,6 @@ static bool flushParams(Tree *t, int pNum)
     stageParams(t, pNum);
     TreePart &tp = t->tp[pNum];
     if (!tp.pendE && !tp.pendI) return true;
-    if (!tp.flushDirect && !uploadArena(tp, tp.offPi)) return false;
     return launchParams(t, pNum);
 }
 
@@ -1191,10 +1184,10 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps)
         return false;
     }
     std::memcpy(tp.hRootPi, mp->comps[rc].val, sizeof(double) * mp->dim);
-    // what the device copy of the arena lacks -- the eigen route's pending
-    // eigensystems and P, changed root frequencies, a rebuilt program -- goes
-    // up in one copy of the covering range (none when nothing changed); then
-    // the parameter kernels; the pruning launch follows on the stream
+    // what the device copy of the arena lacks -- changed root frequencies, a
+    // rebuilt program -- goes up in one copy of the covering range (none when
+    // nothing changed); then the parameter kernels (which read the pinned
+    // arena directly); the pruning launch follows on the stream
     stageParams(t, pNum);
     (void)nKids;
     size_t lo = tp.arenaBytes, hi = 0;
@@ -1203,7 +1196,6 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps)
         hi = std::max(hi, b);
     };
     const size_t piBytes = sizeof(double) * mp->dim;
-    if (!tp.flushDirect && (tp.pendE || tp.pendI)) need(0, tp.offPi);
     if (!tp.rootPiOnDevice || std::memcmp(tp.rootPiUp, tp.hRootPi, piBytes) != 0) need(tp.offPi, tp.offPi + piBytes);
     if (!tp.progOnDevice) need(tp.offProg, tp.offProg + sizeof(int) * tp.progWords);
     if (hi > lo) {
