@@ -537,7 +537,14 @@ static void stageParams(Tree *t, int pNum)
 // Launch the staged eigensystems, then the staged P matrices.  Both read
 // their inputs (slot list, (pi, R) snapshots, branch lengths, nodes, slots)
 // from the pinned arena itself, through its device address: no upload.
-static bool launchParams(Tree *t, int pNum)
+// A 20-state direct-P flush whose launch a batched call makes together
+// with the other items' (one workgroup per matrix either way).
+struct DeferredP {
+    PmatArgs a;
+    const double *in;
+};
+
+static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = nullptr)
 {
     TreePart &tp = t->tp[pNum];
     ModelPart *mp = t->model->parts[pNum];
@@ -556,7 +563,9 @@ static bool launchParams(Tree *t, int pNum)
         PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
                    reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
                    reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
-        if (direct) {
+        if (direct && S == 20 && defer) {
+            defer->push_back({a, reinterpret_cast<const double *>(base)});
+        } else if (direct) {
             if (!hipOk(launchPexp(a, reinterpret_cast<const double *>(base), s), "launch P kernel"))
                 return false;
         } else if (!hipOk(launchPmat(a, s), "launch P kernel")) {
@@ -1171,7 +1180,7 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
 }
 
 // Stage the program and root frequencies of one part (async).
-static bool stagePart(Tree *t, int pNum, bool full, int *nOps)
+static bool stagePart(Tree *t, int pNum, bool full, int *nOps, std::vector<DeferredP> *defer = nullptr)
 {
     TreePart &tp = t->tp[pNum];
     ModelPart *mp = t->model->parts[pNum];
@@ -1206,7 +1215,7 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps)
         tp.rootPiOnDevice = true;
         tp.progOnDevice = true;
     }
-    return launchParams(t, pNum);
+    return launchParams(t, pNum, defer);
 }
 
 static void finishPart(Tree *t, int pNum, bool siteLikes, double lnL, bool bad)
@@ -2184,6 +2193,11 @@ struct p4g_batch {
     PruneArgs *hArgs = nullptr;
     int *hBlockItem = nullptr;
     int capItems = 0, capBlocks = 0;
+    // deferred 20-state direct-P flushes, read by the kernel from pinned memory
+    PmatArgs *hPm = nullptr;
+    const double **hPmIn = nullptr;
+    int *hPmFirst = nullptr;
+    int capPm = 0;
     double *hRes = nullptr;
     ~p4g_batch()
     {
@@ -2192,6 +2206,9 @@ struct p4g_batch {
         hostFree(hArgs);
         hostFree(hBlockItem);
         hostFree(hRes);
+        hostFree(hPm);
+        hostFree(hPmIn);
+        hostFree(hPmFirst);
     }
 };
 
@@ -2222,14 +2239,45 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         int p, nOps, tree;
     };
     std::vector<Item> items;
+    std::vector<DeferredP> defer;
     for (size_t i = 0; i < b->trees.size(); i++) {
         Tree *t = b->trees[i];
         if (!treeReady(t)) return 1;
         for (int p = 0; p < t->nParts; p++) {
             int nOps = 0;
-            if (!stagePart(t, p, false, &nOps)) return 1;
+            if (!stagePart(t, p, false, &nOps, &defer)) return 1;
             items.push_back({t, p, nOps, (int)i});
         }
+    }
+    if (!defer.empty()) {
+        // every deferred 20-state P flush in one launch, before any pruning
+        const int nD = (int)defer.size();
+        if (nD > b->capPm) {
+            hostFree(b->hPm);
+            hostFree(b->hPmIn);
+            hostFree(b->hPmFirst);
+            b->capPm = 2 * nD;
+            if (!hostAlloc(&b->hPm, b->capPm, "hipHostMalloc batch P args") ||
+                !hostAlloc(&b->hPmIn, b->capPm, "hipHostMalloc batch P inputs") ||
+                !hostAlloc(&b->hPmFirst, b->capPm + 1, "hipHostMalloc batch P map"))
+                return 1;
+        }
+        int nBlk = 0;
+        for (int q = 0; q < nD; q++) {
+            b->hPm[q] = defer[q].a;
+            b->hPmIn[q] = defer[q].in;
+            b->hPmFirst[q] = nBlk;
+            nBlk += defer[q].a.nItems * defer[q].a.K;
+        }
+        b->hPmFirst[nD] = nBlk;
+        PmatArgs *dPm = nullptr;
+        const double **dPmIn = nullptr;
+        int *dPmFirst = nullptr;
+        if (!hostDevPtr(b->hPm, &dPm, "device address of batch P args") ||
+            !hostDevPtr(b->hPmIn, &dPmIn, "device address of batch P inputs") ||
+            !hostDevPtr(b->hPmFirst, &dPmFirst, "device address of batch P map") ||
+            !hipOk(launchPexpMulti(dPm, dPmIn, dPmFirst, nD, nBlk, stream()), "launch batched P kernel"))
+            return 1;
     }
     const int nItems = (int)items.size();
     int nBlocksTot = 0;

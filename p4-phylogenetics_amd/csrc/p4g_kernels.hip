@@ -351,7 +351,7 @@ __global__ __launch_bounds__(64) void k_pexp(PmatArgs a, const double *in)
 // owns entries l, l + 64, ...; every product sums over m in the same order
 // as matMul.
 template <int S>
-__global__ __launch_bounds__(64) void k_pexpWave(PmatArgs a, const double *in)
+__device__ __forceinline__ void pexpWaveBody(const PmatArgs &a, const double *in, int it, int k)
 {
     constexpr int NE = S * S;
     constexpr int PER = (NE + 63) / 64;
@@ -359,8 +359,6 @@ __global__ __launch_bounds__(64) void k_pexpWave(PmatArgs a, const double *in)
     __shared__ double red[S];
     __shared__ double sumODE;
     const int lane = threadIdx.x;
-    const int it = blockIdx.x / a.K, k = blockIdx.x - (blockIdx.x / a.K) * a.K;
-    if (it >= a.nItems) return;   // the whole workgroup
     const int node = a.nodeList[it];
     const double t = a.t[(size_t)it * a.K + k];
     const double *pi = in + (size_t)a.eigSlot[it] * (S + NE);
@@ -456,6 +454,38 @@ __global__ __launch_bounds__(64) void k_pexpWave(PmatArgs a, const double *in)
         a.PT64[o + j * S + i] = pij;
         a.PT32[o + j * S + i] = (float)pij;
     }
+}
+
+template <int S>
+__global__ __launch_bounds__(64) void k_pexpWave(PmatArgs a, const double *in)
+{
+    const int it = blockIdx.x / a.K, k = blockIdx.x - (blockIdx.x / a.K) * a.K;
+    if (it >= a.nItems) return;   // the whole workgroup
+    pexpWaveBody<S>(a, in, it, k);
+}
+
+// All (tree, part) items of a batched call in one launch: item i owns
+// workgroups [first[i], first[i + 1]), one per (branch, category).
+template <int S>
+__global__ __launch_bounds__(64) void k_pexpWaveMulti(const PmatArgs *args, const double *const *ins,
+                                                      const int *first, int nItems)
+{
+    const int b = blockIdx.x;
+    int i = 0;
+    while (i + 1 < nItems && first[i + 1] <= b) i++;
+    const PmatArgs a = args[i];
+    const int local = b - first[i];
+    const int it = local / a.K, k = local - (local / a.K) * a.K;
+    if (it >= a.nItems) return;   // the whole workgroup
+    pexpWaveBody<S>(a, ins[i], it, k);
+}
+
+hipError_t launchPexpMulti(const PmatArgs *args, const double *const *ins, const int *first, int nItems,
+                           int nBlocks, hipStream_t s)
+{
+    if (nItems == 0 || nBlocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pexpWaveMulti<20>, dim3(nBlocks), dim3(64), 0, s, args, ins, first, nItems);
+    return hipGetLastError();
 }
 
 static int g_pexpDirect = 1;
