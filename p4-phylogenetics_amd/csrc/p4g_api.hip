@@ -537,8 +537,8 @@ static void stageParams(Tree *t, int pNum)
 // Launch the staged eigensystems, then the staged P matrices.  Both read
 // their inputs (slot list, (pi, R) snapshots, branch lengths, nodes, slots)
 // from the pinned arena itself, through its device address: no upload.
-// A 20-state direct-P flush whose launch a batched call makes together
-// with the other items' (one workgroup per matrix either way).
+// A direct-P flush (4 or 20 states) whose launch a batched call makes
+// together with the other items' of the same state count.
 struct DeferredP {
     PmatArgs a;
     const double *in;
@@ -563,7 +563,7 @@ static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = null
         PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
                    reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
                    reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
-        if (direct && S == 20 && defer) {
+        if (direct && defer) {
             defer->push_back({a, reinterpret_cast<const double *>(base)});
         } else if (direct) {
             if (!hipOk(launchPexp(a, reinterpret_cast<const double *>(base), s), "launch P kernel"))
@@ -2250,7 +2250,8 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         }
     }
     if (!defer.empty()) {
-        // every deferred 20-state P flush in one launch, before any pruning
+        // the deferred direct-P flushes in one launch per state count, before
+        // any pruning; the arrays hold the 20-state items, then the 4-state
         const int nD = (int)defer.size();
         if (nD > b->capPm) {
             hostFree(b->hPm);
@@ -2259,25 +2260,33 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
             b->capPm = 2 * nD;
             if (!hostAlloc(&b->hPm, b->capPm, "hipHostMalloc batch P args") ||
                 !hostAlloc(&b->hPmIn, b->capPm, "hipHostMalloc batch P inputs") ||
-                !hostAlloc(&b->hPmFirst, b->capPm + 1, "hipHostMalloc batch P map"))
+                !hostAlloc(&b->hPmFirst, b->capPm + 2, "hipHostMalloc batch P map"))
                 return 1;
         }
-        int nBlk = 0;
-        for (int q = 0; q < nD; q++) {
-            b->hPm[q] = defer[q].a;
-            b->hPmIn[q] = defer[q].in;
-            b->hPmFirst[q] = nBlk;
-            nBlk += defer[q].a.nItems * defer[q].a.K;
-        }
-        b->hPmFirst[nD] = nBlk;
         PmatArgs *dPm = nullptr;
         const double **dPmIn = nullptr;
         int *dPmFirst = nullptr;
         if (!hostDevPtr(b->hPm, &dPm, "device address of batch P args") ||
             !hostDevPtr(b->hPmIn, &dPmIn, "device address of batch P inputs") ||
-            !hostDevPtr(b->hPmFirst, &dPmFirst, "device address of batch P map") ||
-            !hipOk(launchPexpMulti(dPm, dPmIn, dPmFirst, nD, nBlk, stream()), "launch batched P kernel"))
+            !hostDevPtr(b->hPmFirst, &dPmFirst, "device address of batch P map"))
             return 1;
+        int q = 0, f = 0;
+        for (int dim : {20, 4}) {
+            const int q0 = q, f0 = f;
+            int total = 0;
+            for (const DeferredP &dp : defer)
+                if (dp.a.dim == dim) {
+                    b->hPm[q] = dp.a;
+                    b->hPmIn[q] = dp.in;
+                    b->hPmFirst[f++] = total;
+                    total += dp.a.nItems * dp.a.K;
+                    q++;
+                }
+            b->hPmFirst[f++] = total;
+            if (!hipOk(launchPexpMulti(dim, dPm + q0, dPmIn + q0, dPmFirst + f0, q - q0, total, stream()),
+                       "launch batched P kernel"))
+                return 1;
+        }
     }
     const int nItems = (int)items.size();
     int nBlocksTot = 0;

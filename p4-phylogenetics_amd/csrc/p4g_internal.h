@@ -330,11 +330,11 @@ hipError_t launchPmat(const PmatArgs &a, hipStream_t s);
 // eigIn (4-state parts only); pmatDirect(dim) says whether that route is on.
 hipError_t launchPexp(const PmatArgs &a, const double *eigIn, hipStream_t s);
 bool pmatDirect(int dim);
-// 20-state direct P for several items in one launch: args/ins/first are
-// device-readable arrays of nItems entries (first: nItems + 1 prefix sums of
-// nItems * K workgroups per item, first[nItems] == nBlocks).
-hipError_t launchPexpMulti(const PmatArgs *args, const double *const *ins, const int *first, int nItems,
-                           int nBlocks, hipStream_t s);
+// Direct P (dim 4 or 20) for several items in one launch: args/ins/first
+// are device-readable arrays of nItems entries (first: nItems + 1 prefix
+// sums of each item's nItems * K matrices, first[nItems] == total).
+hipError_t launchPexpMulti(int dim, const PmatArgs *args, const double *const *ins, const int *first, int nItems,
+                           int total, hipStream_t s);
 // 20-state flushes with fewer pending P matrices use the eigen route.
 constexpr int PEXP_WAVE_MIN_ITEMS = 16;
 void setPmatRoute(int direct);

@@ -268,11 +268,8 @@ __device__ __forceinline__ void matMul(const double (&X)[S][S], const double (&Y
 }
 
 template <int S>
-__global__ __launch_bounds__(64) void k_pexp(PmatArgs a, const double *in)
+__device__ __forceinline__ void pexpBody(const PmatArgs &a, const double *in, int it, int k)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= a.nItems * a.K) return;
-    const int it = gid / a.K, k = gid % a.K;
     const int node = a.nodeList[it];
     const double t = a.t[(size_t)it * a.K + k];
     const double *pi = in + (size_t)a.eigSlot[it] * (S + S * S);
@@ -344,6 +341,34 @@ __global__ __launch_bounds__(64) void k_pexp(PmatArgs a, const double *in)
             a.PT64[o + j * S + i] = pij;
             a.PT32[o + j * S + i] = (float)pij;
         }
+}
+
+template <int S>
+__global__ __launch_bounds__(64) void k_pexp(PmatArgs a, const double *in)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= a.nItems * a.K) return;
+    pexpBody<S>(a, in, gid / a.K, gid % a.K);
+}
+
+// All items of a batched call in one launch: item i owns threads
+// [first[i], first[i + 1]), one per (branch, category).
+template <int S>
+__global__ __launch_bounds__(64) void k_pexpMulti(const PmatArgs *args, const double *const *ins, const int *first,
+                                                  int nItems, int total)
+{
+    // the item map (pinned host memory) is read once, in parallel, into LDS
+    extern __shared__ int sFirst[];
+    for (int j = threadIdx.x; j <= nItems; j += blockDim.x) sFirst[j] = first[j];
+    __syncthreads();
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    int i = 0;
+    while (i + 1 < nItems && sFirst[i + 1] <= gid) i++;
+    const PmatArgs a = args[i];
+    const int local = gid - sFirst[i];
+    if (local >= a.nItems * a.K) return;
+    pexpBody<S>(a, ins[i], local / a.K, local % a.K);
 }
 
 // The same series and doublings for larger state counts (20: proteins),
@@ -470,21 +495,31 @@ template <int S>
 __global__ __launch_bounds__(64) void k_pexpWaveMulti(const PmatArgs *args, const double *const *ins,
                                                       const int *first, int nItems)
 {
+    extern __shared__ int sFirst[];
+    for (int j = threadIdx.x; j <= nItems; j += blockDim.x) sFirst[j] = first[j];
+    __syncthreads();
     const int b = blockIdx.x;
     int i = 0;
-    while (i + 1 < nItems && first[i + 1] <= b) i++;
+    while (i + 1 < nItems && sFirst[i + 1] <= b) i++;
     const PmatArgs a = args[i];
-    const int local = b - first[i];
+    const int local = b - sFirst[i];
     const int it = local / a.K, k = local - (local / a.K) * a.K;
     if (it >= a.nItems) return;   // the whole workgroup
     pexpWaveBody<S>(a, ins[i], it, k);
 }
 
-hipError_t launchPexpMulti(const PmatArgs *args, const double *const *ins, const int *first, int nItems,
-                           int nBlocks, hipStream_t s)
+hipError_t launchPexpMulti(int dim, const PmatArgs *args, const double *const *ins, const int *first, int nItems,
+                           int total, hipStream_t s)
 {
-    if (nItems == 0 || nBlocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pexpWaveMulti<20>, dim3(nBlocks), dim3(64), 0, s, args, ins, first, nItems);
+    if (nItems == 0 || total == 0) return hipSuccess;
+    const size_t lds = sizeof(int) * ((size_t)nItems + 1);   // the item map
+    if (lds > 32 * 1024) return hipErrorInvalidValue;
+    if (dim == 20)
+        hipLaunchKernelGGL(k_pexpWaveMulti<20>, dim3(total), dim3(64), lds, s, args, ins, first, nItems);
+    else if (dim == 4)
+        hipLaunchKernelGGL(k_pexpMulti<4>, dim3((total + 63) / 64), dim3(64), lds, s, args, ins, first, nItems, total);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

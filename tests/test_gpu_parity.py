@@ -394,20 +394,18 @@ def test_batch_mixes_protein_kernels():
 
 def test_batch_direct_p_in_one_launch():
     """setPrams on several multi-partition chains, then one batched call:
-    the 20-state P matrices of every chain come from one launch and equal,
-    bit for bit, those of single-tree evaluation; every lnL equals the
-    tree's full evaluation."""
+    the 4- and 20-state P matrices of every chain come from one launch per
+    state count and equal, bit for bit, those of single-tree evaluation;
+    every lnL equals the tree's full evaluation."""
     trees = [synth.make_workload("multi48", seed=s, nSites=1500, precision="fp32")[0] for s in (8, 9, 10)]
     want = [t.calcLogLike(verbose=0) for t in trees]
     decks = []
     for t in trees:
         d = {}
         for p, mp in enumerate(t.model.parts):
-            if mp.dim != 20:
-                continue
             for n in list(t.iterNodesNoRoot())[:6]:
                 d[(p, n.nodeNum)] = pf.p4_getBigPDeck(n.cNode, p, mp.nGammaCat, mp.dim)
-        assert d
+        assert {t.model.parts[p].dim for p, _ in d} == {4, 20}
         decks.append(d)
     for t in trees:
         pf.p4_setPrams(t.cTree, -1)
