@@ -541,7 +541,7 @@ static void stageParams(Tree *t, int pNum)
 // together with the other items' of the same state count.
 struct DeferredP {
     PmatArgs a;
-    const double *in;
+    const double *in;   // (pi, R) snapshots of the direct route; null: eigen route
 };
 
 static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = nullptr)
@@ -568,6 +568,8 @@ static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = null
         } else if (direct) {
             if (!hipOk(launchPexp(a, reinterpret_cast<const double *>(base), s), "launch P kernel"))
                 return false;
+        } else if (defer) {
+            defer->push_back({a, nullptr});
         } else if (!hipOk(launchPmat(a, s), "launch P kernel")) {
             return false;
         }
@@ -2260,7 +2262,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
             b->capPm = 2 * nD;
             if (!hostAlloc(&b->hPm, b->capPm, "hipHostMalloc batch P args") ||
                 !hostAlloc(&b->hPmIn, b->capPm, "hipHostMalloc batch P inputs") ||
-                !hostAlloc(&b->hPmFirst, b->capPm + 2, "hipHostMalloc batch P map"))
+                !hostAlloc(&b->hPmFirst, b->capPm + 3, "hipHostMalloc batch P map"))
                 return 1;
         }
         PmatArgs *dPm = nullptr;
@@ -2275,7 +2277,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
             const int q0 = q, f0 = f;
             int total = 0;
             for (const DeferredP &dp : defer)
-                if (dp.a.dim == dim) {
+                if (dp.in && dp.a.dim == dim) {
                     b->hPm[q] = dp.a;
                     b->hPmIn[q] = dp.in;
                     b->hPmFirst[f++] = total;
@@ -2287,6 +2289,20 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
                        "launch batched P kernel"))
                 return 1;
         }
+        // the eigen route's items (after their eigensystem launches)
+        const int q0 = q, f0 = f;
+        int total = 0;
+        for (const DeferredP &dp : defer)
+            if (!dp.in) {
+                b->hPm[q] = dp.a;
+                b->hPmIn[q] = nullptr;
+                b->hPmFirst[f++] = total;
+                total += dp.a.nItems * dp.a.K * dp.a.dim * dp.a.dim;
+                q++;
+            }
+        b->hPmFirst[f++] = total;
+        if (!hipOk(launchPmatMulti(dPm + q0, dPmFirst + f0, q - q0, total, stream()), "launch batched P kernel"))
+            return 1;
     }
     const int nItems = (int)items.size();
     int nBlocksTot = 0;

@@ -326,6 +326,9 @@ struct PruneArgs {
 // Kernel launchers (p4g_kernels.hip).
 hipError_t launchEig(const EigArgs &a, hipStream_t s);
 hipError_t launchPmat(const PmatArgs &a, hipStream_t s);
+// The eigen route's P for several items in one launch (first: nItems + 1
+// prefix sums of each item's nItems * K * dim * dim entries).
+hipError_t launchPmatMulti(const PmatArgs *args, const int *first, int nItems, int total, hipStream_t s);
 // P = exp(Q t) per (branch, category) straight from the (pi, R) snapshots
 // eigIn (4-state parts only); pmatDirect(dim) says whether that route is on.
 hipError_t launchPexp(const PmatArgs &a, const double *eigIn, hipStream_t s);

@@ -209,12 +209,9 @@ hipError_t launchEig(const EigArgs &a, hipStream_t s)
 }
 
 // ------------------------------------------------------------- P matrices
-__global__ __launch_bounds__(256) void k_pmat(PmatArgs a)
+__device__ __forceinline__ void pmatBody(const PmatArgs &a, long gid)
 {
     const int S = a.dim, K = a.K;
-    const long total = (long)a.nItems * K * S * S;
-    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= total) return;
     const int j = gid % S;
     const int i = (gid / S) % S;
     const int k = (gid / (S * S)) % K;
@@ -231,6 +228,41 @@ __global__ __launch_bounds__(256) void k_pmat(PmatArgs a)
     const size_t o = (((size_t)node * K + k) * S + j) * S + i;
     a.PT64[o] = pij;
     a.PT32[o] = (float)pij;
+}
+
+__global__ __launch_bounds__(256) void k_pmat(PmatArgs a)
+{
+    const long total = (long)a.nItems * a.K * a.dim * a.dim;
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    pmatBody(a, gid);
+}
+
+// The eigen route's P matrices of all items of a batched call in one
+// launch: item i owns entries [first[i], first[i + 1]), one thread each;
+// the item map is read once into LDS.
+__global__ __launch_bounds__(256) void k_pmatMulti(const PmatArgs *args, const int *first, int nItems, int total)
+{
+    extern __shared__ int sFirst[];
+    for (int j = threadIdx.x; j <= nItems; j += blockDim.x) sFirst[j] = first[j];
+    __syncthreads();
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= total) return;
+    int i = 0;
+    while (i + 1 < nItems && sFirst[i + 1] <= gid) i++;
+    const PmatArgs a = args[i];
+    const int local = gid - sFirst[i];
+    if ((long)local >= (long)a.nItems * a.K * a.dim * a.dim) return;
+    pmatBody(a, local);
+}
+
+hipError_t launchPmatMulti(const PmatArgs *args, const int *first, int nItems, int total, hipStream_t s)
+{
+    if (nItems == 0 || total == 0) return hipSuccess;
+    const size_t lds = sizeof(int) * ((size_t)nItems + 1);
+    if (lds > 32 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pmatMulti, dim3((total + 255) / 256), dim3(256), lds, s, args, first, nItems, total);
+    return hipGetLastError();
 }
 
 hipError_t launchPmat(const PmatArgs &a, hipStream_t s)

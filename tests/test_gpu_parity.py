@@ -419,3 +419,42 @@ def test_batch_direct_p_in_one_launch():
             mp = t.model.parts[p]
             after = pf.p4_getBigPDeck(t.nodes[nn].cNode, p, mp.nGammaCat, mp.dim)
             assert numpy.array_equal(after, before), (p, nn)
+
+
+def test_batch_eigen_route_p_in_one_launch():
+    """A branch move on every chain of a multi-partition batch: 20-state
+    parts recompute that branch's P on the eigen route (a flush of few
+    matrices), all items in one launch; P decks equal, bit for bit, and lnL
+    agrees with the same trees evaluated one by one."""
+    def make():
+        return [synth.make_workload("multi48", seed=s, nSites=1500, precision="fp32")[0] for s in (11, 12)]
+
+    def move(t):
+        n = list(t.iterNodesNoRoot())[5]
+        pf.p4_setBrLen(n.cNode, n.br.len * 1.3)
+        pf.p4_calculateBigPDecks(n.cNode)
+        m = n.parent
+        while m is not None:
+            for p in range(t.model.nParts):
+                pf.p4_setConditionalLikelihoodsOfInternalNodePart(m.cNode, p)
+            m = m.parent
+        return n.nodeNum
+
+    A, B = make(), make()
+    for t in A + B:
+        t.calcLogLike(verbose=0)
+    moved = [move(t) for t in A]
+    for t in B:
+        move(t)
+    got = pf.Batch([t.cTree for t in A]).logLikes()
+    want = [pf.p4_treeLogLike(t.cTree, 0) for t in B]
+    for g, w in zip(got, want):
+        assert rel(g, w) < 1e-12, (g, w)
+    for ta, tb, nn in zip(A, B, moved):
+        dims = set()
+        for p, mp in enumerate(ta.model.parts):
+            pa = pf.p4_getBigPDeck(ta.nodes[nn].cNode, p, mp.nGammaCat, mp.dim)
+            pb = pf.p4_getBigPDeck(tb.nodes[nn].cNode, p, mp.nGammaCat, mp.dim)
+            assert numpy.array_equal(pa, pb), (p, nn)
+            dims.add(mp.dim)
+        assert dims == {4, 20}
