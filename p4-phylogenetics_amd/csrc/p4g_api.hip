@@ -544,6 +544,19 @@ struct DeferredP {
     const double *in;   // (pi, R) snapshots of the direct route; null: eigen route
 };
 
+// Error path of a batched call: the P flushes deferred so far are launched
+// one by one, so the items already staged (their marks cleared) keep
+// consistent P matrices; the first error stays the reported one.
+static void launchDeferredEach(const std::vector<DeferredP> &defer)
+{
+    for (const DeferredP &dp : defer) {
+        if (dp.in)
+            (void)launchPexp(dp.a, dp.in, stream());
+        else
+            (void)launchPmat(dp.a, stream());
+    }
+}
+
 static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = nullptr)
 {
     TreePart &tp = t->tp[pNum];
@@ -2244,10 +2257,16 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     std::vector<DeferredP> defer;
     for (size_t i = 0; i < b->trees.size(); i++) {
         Tree *t = b->trees[i];
-        if (!treeReady(t)) return 1;
+        if (!treeReady(t)) {
+            launchDeferredEach(defer);
+            return 1;
+        }
         for (int p = 0; p < t->nParts; p++) {
             int nOps = 0;
-            if (!stagePart(t, p, false, &nOps, &defer)) return 1;
+            if (!stagePart(t, p, false, &nOps, &defer)) {
+                launchDeferredEach(defer);
+                return 1;
+            }
             items.push_back({t, p, nOps, (int)i});
         }
     }
@@ -2262,16 +2281,21 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
             b->capPm = 2 * nD;
             if (!hostAlloc(&b->hPm, b->capPm, "hipHostMalloc batch P args") ||
                 !hostAlloc(&b->hPmIn, b->capPm, "hipHostMalloc batch P inputs") ||
-                !hostAlloc(&b->hPmFirst, b->capPm + 3, "hipHostMalloc batch P map"))
+                !hostAlloc(&b->hPmFirst, b->capPm + 3, "hipHostMalloc batch P map")) {
+                b->capPm = 0;
+                launchDeferredEach(defer);
                 return 1;
+            }
         }
         PmatArgs *dPm = nullptr;
         const double **dPmIn = nullptr;
         int *dPmFirst = nullptr;
         if (!hostDevPtr(b->hPm, &dPm, "device address of batch P args") ||
             !hostDevPtr(b->hPmIn, &dPmIn, "device address of batch P inputs") ||
-            !hostDevPtr(b->hPmFirst, &dPmFirst, "device address of batch P map"))
+            !hostDevPtr(b->hPmFirst, &dPmFirst, "device address of batch P map")) {
+            launchDeferredEach(defer);
             return 1;
+        }
         int q = 0, f = 0;
         for (int dim : {20, 4}) {
             const int q0 = q, f0 = f;

@@ -458,3 +458,36 @@ def test_batch_eigen_route_p_in_one_launch():
             assert numpy.array_equal(pa, pb), (p, nn)
             dims.add(mp.dim)
         assert dims == {4, 20}
+
+
+def test_batch_error_keeps_staged_p_matrices():
+    """A batched call whose second tree fails to stage (a postorder holding
+    an out-of-range node) still computes the P matrices it deferred for the
+    first tree: they equal those of the same move evaluated alone."""
+    def make():
+        return synth.make_workload("multi48", seed=13, nSites=1500, precision="fp32")[0]
+
+    def move(t):
+        n = list(t.iterNodesNoRoot())[5]
+        pf.p4_setBrLen(n.cNode, n.br.len * 1.7)
+        pf.p4_calculateBigPDecks(n.cNode)
+        return n.nodeNum
+
+    a, ctrl = make(), make()
+    bad = synth.make_workload("gtr16", seed=2, nSites=300, precision="fp32")[0]
+    for t in (a, ctrl, bad):
+        t.calcLogLike(verbose=0)
+    nn = move(a)
+    move(ctrl)
+    saved = int(bad.postOrder[0])
+    bad.postOrder[0] = 10 ** 6
+    try:
+        with pytest.raises(P4Error, match="postOrder"):
+            pf.Batch([a.cTree, bad.cTree]).logLikes()
+    finally:
+        bad.postOrder[0] = saved
+    pf.p4_treeLogLike(ctrl.cTree, 0)
+    for p, mp in enumerate(a.model.parts):
+        got = pf.p4_getBigPDeck(a.nodes[nn].cNode, p, mp.nGammaCat, mp.dim)
+        want = pf.p4_getBigPDeck(ctrl.nodes[nn].cNode, p, mp.nGammaCat, mp.dim)
+        assert numpy.array_equal(got, want), p
