@@ -95,8 +95,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a plain device copy of known size (for PMC scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (site patterns would be "
@@ -131,18 +129,6 @@ def main():
     if os.path.exists(tf):
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
-
-    if args.calibrate_copy:
-        # an elementwise kernel of known traffic (256 MiB read, 256 MiB
-        # written) for scaling the FETCH_SIZE / WRITE_SIZE counters; a
-        # device-to-device copy_ may go to a DMA engine the counters miss
-        import torch
-        n = 256 << 20
-        a = torch.ones(n // 4, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(5):
-            torch.add(a, 1.0, out=b)
-        torch.cuda.synchronize()
 
     out = {
         "metric": "log-like evals/s (50k-site DNA, 1 GPU)",

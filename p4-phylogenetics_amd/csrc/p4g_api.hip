@@ -555,6 +555,9 @@ static void launchDeferredEach(const std::vector<DeferredP> &defer)
         else
             (void)launchPmat(dp.a, stream());
     }
+    // the kernels read the pinned staging arena, which the next staging call
+    // overwrites: they must have run before this error path returns
+    (void)hipStreamSynchronize(stream());
 }
 
 static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = nullptr)
@@ -1263,15 +1266,20 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
     const hipError_t le = tp.wave ? launchPruneWave(a, t->precision, nBlocks, tp.waveLds, s)
                           : pruneUsesMfma(a.dim, a.K, t->precision) ? launchPruneAa(a, nBlocks, s)
                                                                     : launchPrune(a, t->precision, nBlocks, s);
-    if (!hipOk(le, "launch pruning kernel")) return false;
+    if (!hipOk(le, "launch pruning kernel")) {
+        (void)hipStreamSynchronize(s);  // queued P kernels read the staging arena
+        return false;
+    }
     if (g_timing) (void)hipEventRecord(tp.ev1, s);
     std::vector<double> pl;
     if (siteLikes) {
         pl.resize(tp.nPat);
         if (!hipOk(hipMemcpyAsync(pl.data(), tp.patLikes, sizeof(double) * tp.nPat,
                                   hipMemcpyDeviceToHost, s),
-                   "copy site likes"))
+                   "copy site likes")) {
+            (void)hipStreamSynchronize(s);
             return false;
+        }
     }
     if (!hipOk(hipStreamSynchronize(s), "pruning kernel")) return false;
     if (g_timing) {
@@ -2270,6 +2278,12 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
             items.push_back({t, p, nOps, (int)i});
         }
     }
+    // error paths once kernels are queued: they read pinned staging memory
+    // (arena, batch argument arrays) that the next call rewrites, so wait
+    auto failQueued = [&]() {
+        (void)hipStreamSynchronize(stream());
+        return 1;
+    };
     if (!defer.empty()) {
         // the deferred direct-P flushes in one launch per state count, before
         // any pruning; the arrays hold the 20-state items, then the 4-state
@@ -2311,7 +2325,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
             b->hPmFirst[f++] = total;
             if (!hipOk(launchPexpMulti(dim, dPm + q0, dPmIn + q0, dPmFirst + f0, q - q0, total, stream()),
                        "launch batched P kernel"))
-                return 1;
+                return failQueued();
         }
         // the eigen route's items (after their eigensystem launches)
         const int q0 = q, f0 = f;
@@ -2326,7 +2340,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
             }
         b->hPmFirst[f++] = total;
         if (!hipOk(launchPmatMulti(dPm + q0, dPmFirst + f0, q - q0, total, stream()), "launch batched P kernel"))
-            return 1;
+            return failQueued();
     }
     const int nItems = (int)items.size();
     int nBlocksTot = 0;
@@ -2345,7 +2359,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         if (!devAlloc(&b->dArgs, b->capItems, "hipMalloc batch args") ||
             !hostAlloc(&b->hArgs, b->capItems, "hipHostMalloc batch args") ||
             !hostAlloc(&b->hRes, (size_t)b->capItems * 2, "hipHostMalloc batch results"))
-            return 1;
+            return failQueued();
     }
     if (nBlocksTot > b->capBlocks) {
         devFree(b->dBlockItem);
@@ -2353,7 +2367,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         b->capBlocks = nBlocksTot * 2;
         if (!devAlloc(&b->dBlockItem, (size_t)b->capBlocks * 2, "hipMalloc batch map") ||
             !hostAlloc(&b->hBlockItem, (size_t)b->capBlocks * 2, "hipHostMalloc batch map"))
-            return 1;
+            return failQueued();
     }
     // group by (dim, precision): items of a group are contiguous in hArgs
     std::vector<int> order(nItems);
@@ -2373,7 +2387,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
     double *dRes = nullptr;
-    if (!hostDevPtr(b->hRes, &dRes, "device address of batch results")) return 1;
+    if (!hostDevPtr(b->hRes, &dRes, "device address of batch results")) return failQueued();
     for (int j = 0; j < nItems; j++) {
         const Item &it = items[order[j]];
         fillPruneArgs(it.t, it.p, it.nOps, b->hArgs[j], false);
@@ -2382,7 +2396,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     hipStream_t s = stream();
     if (!hipOk(hipMemcpyAsync(b->dArgs, b->hArgs, sizeof(PruneArgs) * nItems, hipMemcpyHostToDevice, s),
                "stage batch args"))
-        return 1;
+        return failQueued();
     int j = 0, blk = 0;
     std::vector<std::pair<int, int>> launches;  // (first block, count) per group
     std::vector<int> groupDim, groupPrec;
@@ -2429,7 +2443,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     }
     if (!hipOk(hipMemcpyAsync(b->dBlockItem, b->hBlockItem, sizeof(int) * 2 * blk, hipMemcpyHostToDevice, s),
                "stage batch map"))
-        return 1;
+        return failQueued();
     // P^T of each wave item's children in program order
     for (int q = 0; q < nItems; q++) {
         const Item &it = items[order[q]];
@@ -2438,7 +2452,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         const PruneArgs &pa = b->hArgs[q];
         if (!hipOk(launchPgather(pa.PT, pa.wpre + 3 * pa.nPre, tp.nKidTot, tp.K, tp.PP, it.t->precision, s),
                    "launch P gather"))
-            return 1;
+            return failQueued();
     }
     for (size_t g = 0; g < launches.size(); g++) {
         const int *bi = b->dBlockItem + 2 * launches[g].first;
@@ -2448,7 +2462,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
                                   ? launchPruneAaBatch(b->dArgs, bi, launches[g].second, groupK[g], groupAaCat[g] != 0, s)
                                   : launchPruneBatch(b->dArgs, bi, launches[g].second, groupDim[g], groupPrec[g],
                                                      groupLds[g], s);
-        if (!hipOk(le, "launch batched pruning kernel")) return 1;
+        if (!hipOk(le, "launch batched pruning kernel")) return failQueued();
     }
     if (!hipOk(hipStreamSynchronize(s), "batched pruning kernel")) return 1;
     for (size_t i = 0; i < b->trees.size(); i++) lnL[i] = 0.0;

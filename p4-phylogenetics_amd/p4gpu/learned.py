@@ -450,6 +450,12 @@ class LearnedProposal(object):
         self._setLens(ch, T, new)
         mp.gdasrvs[0].val = alpha2
         ch._modelChanged(self.pNum)
+        # every branch length changed: the other parts' P matrices and
+        # partials are stale too (their model parameters are not)
+        for p in range(t.model.nParts):
+            if p != self.pNum:
+                pf.p4_setPrams(t.cTree, p)
+                ch._markAll(t, p)
 
     def _block(self, ch):
         t = ch.propTree

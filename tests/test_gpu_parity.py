@@ -393,32 +393,52 @@ def test_batch_mixes_protein_kernels():
 
 
 def test_batch_direct_p_in_one_launch():
-    """setPrams on several multi-partition chains, then one batched call:
-    the 4- and 20-state P matrices of every chain come from one launch per
-    state count and equal, bit for bit, those of single-tree evaluation;
-    every lnL equals the tree's full evaluation."""
-    trees = [synth.make_workload("multi48", seed=s, nSites=1500, precision="fp32")[0] for s in (8, 9, 10)]
-    want = [t.calcLogLike(verbose=0) for t in trees]
-    decks = []
-    for t in trees:
-        d = {}
-        for p, mp in enumerate(t.model.parts):
-            for n in list(t.iterNodesNoRoot())[:6]:
-                d[(p, n.nodeNum)] = pf.p4_getBigPDeck(n.cNode, p, mp.nGammaCat, mp.dim)
-        assert {t.model.parts[p].dim for p, _ in d} == {4, 20}
-        decks.append(d)
-    for t in trees:
+    """Every branch length of several multi-partition chains scaled by 1.3,
+    then setPrams and one batched call: the 4- and 20-state P matrices of
+    every chain come from one launch per state count.  They must equal, bit
+    for bit, those of the same trees evaluated one by one after the same
+    change (so a batched P launch that skipped or misrouted an item would
+    leave the old, different matrices), and every lnL must equal the
+    single-tree evaluation."""
+    def make():
+        return [synth.make_workload("multi48", seed=s, nSites=1500, precision="fp32")[0] for s in (8, 9, 10)]
+
+    def scale(t):
+        for n in t.iterNodesNoRoot():
+            n.br.len *= 1.3
+            pf.p4_setBrLen(n.cNode, n.br.len)
+
+    A, B = make(), make()
+    before = []
+    for t in A + B:
+        t.calcLogLike(verbose=0)
+    for t in A:
+        mp0 = {p: mp for p, mp in enumerate(t.model.parts)}
+        before.append({(p, n.nodeNum): pf.p4_getBigPDeck(n.cNode, p, mp.nGammaCat, mp.dim)
+                       for p, mp in mp0.items() for n in t.iterNodesNoRoot()})
+    for t in A:
+        scale(t)
         pf.p4_setPrams(t.cTree, -1)
         for n in t.iterInternals():
             pf.p4_setConditionalLikelihoodsOfInternalNode(n.cNode)
-    got = pf.Batch([t.cTree for t in trees]).logLikes()
+    got = pf.Batch([t.cTree for t in A]).logLikes()
+    want = []
+    for t in B:
+        scale(t)
+        pf.p4_setPrams(t.cTree, -1)
+        want.append(pf.p4_treeLogLike(t.cTree, 0))
     for g, w in zip(got, want):
         assert rel(g, w) < 1e-12, (g, w)
-    for t, d in zip(trees, decks):
-        for (p, nn), before in d.items():
-            mp = t.model.parts[p]
-            after = pf.p4_getBigPDeck(t.nodes[nn].cNode, p, mp.nGammaCat, mp.dim)
-            assert numpy.array_equal(after, before), (p, nn)
+    for ta, tb, b4 in zip(A, B, before):
+        dims = set()
+        for (p, nn), old in b4.items():
+            mp = ta.model.parts[p]
+            pa = pf.p4_getBigPDeck(ta.nodes[nn].cNode, p, mp.nGammaCat, mp.dim)
+            pb = pf.p4_getBigPDeck(tb.nodes[nn].cNode, p, mp.nGammaCat, mp.dim)
+            assert numpy.array_equal(pa, pb), (p, nn)
+            assert not numpy.array_equal(pa, old), (p, nn)
+            dims.add(mp.dim)
+        assert dims == {4, 20}
 
 
 def test_batch_eigen_route_p_in_one_launch():

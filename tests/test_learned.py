@@ -275,3 +275,31 @@ def test_trained_moves_gain_ess_over_plain():
         se = math.hypot(xp.std() / math.sqrt(eP), xl.std() / math.sqrt(eL))
         assert abs(xl.mean() - xp.mean()) < 5.0 * se, (key, xl.mean(), xp.mean(), se)
     assert min(ratios.values()) > 2.0, (ratios, L.stats)
+
+
+@pytest.mark.gpu
+def test_ridge_move_on_a_multipart_tree_refreshes_every_part():
+    """The ridge move scales every branch length.  On a tree with several
+    partitions the batched lnL of the proposed tree must equal a full
+    recomputation of every part (setPrams(-1) + treeLogLike), so the parts
+    the move's own part does not cover are refreshed too."""
+    tree, _ = synth.make_workload("multi48", seed=4, nSites=800, precision="fp64")
+    m = Mcmc(tree, nChains=1, seed=3, sampleInterval=10, verbose=False)
+    L = _untrained_move(tree, seed=3)
+    L.kinds = ['ridge']
+    L.cum = numpy.array([1.0])
+    ch = m.chains[0]
+    n = 0
+    for _ in range(20):
+        L.propose(ch, None)
+        if ch.logProposalRatio <= -1.0e299:   # out of bounds: nothing changed
+            ch.sync(ch.curTree, ch.propTree)
+            continue
+        got = m._batch.logLikes()[0]
+        t = ch.propTree
+        pf.p4_setPrams(t.cTree, -1)
+        want = pf.p4_treeLogLike(t.cTree, 0)
+        assert abs(got - want) <= 1e-10 * abs(want), (got, want)
+        ch.sync(ch.curTree, ch.propTree)
+        n += 1
+    assert n >= 5
