@@ -166,6 +166,18 @@ int p4g_getPrecision(const p4g_tree *t);
 int p4g_setPrams(p4g_tree *t, int pNum);
 /* pf.p4_treeLogLike (Pf/pfmodule.c:2132-2144, Pf/p4_tree.c:868-922). */
 int p4g_treeLogLike(p4g_tree *t, int getSiteLikes, double *lnL);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving or MCMC loops,
+ * harnesses that forbid hipMalloc): p4g_treeLogLike(t, 0, lnL) computed with
+ * no allocation.  The pruning kernels' reduction scratch (per-workgroup sums,
+ * bad-site flags, the ticket counter) comes from one library-wide device
+ * array of 512 KiB per GPU instead of the tree's own buffers; a part needing
+ * more (more than about 43,000 pattern tiles, some 700,000 patterns) is
+ * rejected with an error.  The tree must already hold device state for its
+ * current precision (one earlier p4g_treeLogLike, which allocates);
+ * otherwise the call fails and allocates nothing.  Calls that overlap on
+ * one GPU race on the shared array. */
+int p4g_treeLogLikeNoAlloc(p4g_tree *t, double *lnL);
 /* pf.p4_partLogLike (Pf/pfmodule.c:2147-2160, Pf/p4_tree.c:924-1027).
  * Recomputes the partials of nodes marked by
  * p4g_setConditionalLikelihoodsOfInternalNodePart (and of all internal

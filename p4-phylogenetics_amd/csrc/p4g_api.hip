@@ -41,6 +41,17 @@ bool hipOk(hipError_t e, const char *what)
 static hipStream_t g_stream = nullptr;
 static int g_devState = 0;  // 0 unknown, 1 ready, -1 unavailable
 
+// Scratch memory of the entry point that allocates nothing
+// (p4g_treeLogLikeNoAlloc): the pruning kernels' per-workgroup partial sums,
+// bad-site flags and ticket counter live here instead of in the tree's own
+// buffers.  Every GPU holds its own copy of the array until the process
+// ends; host code uses the device address that hipGetSymbolAddress returns
+// for the current device (g_scratch), never the array's name.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 512 * 1024;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static unsigned char *g_scratch[MAX_DEVICES] = {};
+
 bool deviceReady()
 {
     if (g_devState == 1) return true;
@@ -62,6 +73,16 @@ bool deviceReady()
         g_devState = -1;
         return false;
     }
+    // the no-allocation entry point's scratch array, for this device
+    int dev = 0;
+    void *ptr = nullptr;
+    if (!hipOk(hipGetDevice(&dev), "hipGetDevice") ||
+        !hipOk(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)") || dev < 0 ||
+        dev >= MAX_DEVICES) {
+        g_devState = -1;
+        return false;
+    }
+    g_scratch[dev] = static_cast<unsigned char *>(ptr);
     g_devState = 1;
     return true;
 }
@@ -1246,8 +1267,12 @@ static void finishPart(Tree *t, int pNum, bool siteLikes, double lnL, bool bad)
     (void)bad;
 }
 
-// Evaluate pending partials + site sum of one part (one launch).
-static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
+// Evaluate pending partials + site sum of one part (one launch).  With
+// `scr` (a device address of the no-allocation scratch array) the pruning
+// kernel's reduction scratch is taken from there, the counter is cleared by
+// a fill kernel first, and the wave kernel reads P^T by node (no staged
+// program-order P stream).
+static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL, unsigned char *scr = nullptr)
 {
     TreePart &tp = t->tp[pNum];
     Part *dp = t->data->parts[pNum];
@@ -1257,7 +1282,27 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL)
     fillPruneArgs(t, pNum, nOps, a, siteLikes);
     hipStream_t s = stream();
     const int nBlocks = tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision, nOps);
-    if (tp.wave && pruneWaveVectorP(t->precision)) {
+    if (scr) {
+        // [counter, 256 B][flags int[nBlocks]][sums double[nBlocks]]
+        const size_t offBad = 256, offSums = offBad + (((size_t)nBlocks * sizeof(int) + 255) & ~(size_t)255);
+        const size_t need = offSums + (size_t)nBlocks * sizeof(double);
+        if (need > NOALLOC_SCRATCH_BYTES) {
+            (void)hipStreamSynchronize(s);  // queued P kernels read the staging arena
+            setError("treeLogLikeNoAlloc: part " + std::to_string(pNum) + " needs " + std::to_string(need) +
+                     " bytes of scratch, more than the " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                     " the library holds (" + std::to_string(tp.nPat) + " patterns); use p4g_treeLogLike");
+            return false;
+        }
+        a.counter = reinterpret_cast<unsigned *>(scr);
+        a.blockBad = reinterpret_cast<int *>(scr + offBad);
+        a.blockSums = reinterpret_cast<double *>(scr + offSums);
+        a.PP = nullptr;
+        if (!hipOk(launchFillU32(a.counter, 0u, 1, s), "clear scratch counter")) {
+            (void)hipStreamSynchronize(s);
+            return false;
+        }
+    }
+    if (tp.wave && pruneWaveVectorP(t->precision) && a.PP) {
         const int *kidP = a.wpre + 3 * tp.nPre;
         if (!hipOk(launchPgather(a.PT, kidP, tp.nKidTot, tp.K, tp.PP, t->precision, s), "launch P gather"))
             return false;
@@ -1904,6 +1949,51 @@ int p4g_treeLogLike(p4g_tree *t, int getSiteLikes, double *lnL)
     for (int p = 0; p < t->nParts; p++) {
         double v = 0.0;
         if (!evalPart(t, p, true, getSiteLikes != 0, &v)) return 1;
+        sum += v;
+    }
+    t->logLike = sum;
+    *lnL = sum;
+    return 0;
+}
+
+int p4g_treeLogLikeNoAlloc(p4g_tree *t, double *lnL)
+{
+    clearError();
+    if (!t || !lnL) {
+        setError("treeLogLikeNoAlloc: bad arguments");
+        return 1;
+    }
+    if (!deviceReady()) return 1;
+    int dev = 0;
+    if (!hipOk(hipGetDevice(&dev), "hipGetDevice")) return 1;
+    unsigned char *scr = (dev >= 0 && dev < MAX_DEVICES) ? g_scratch[dev] : nullptr;
+    if (!scr) {
+        setError("treeLogLikeNoAlloc: no scratch address for the current device");
+        return 1;
+    }
+    // the tree's own device state must exist already: this call sets up
+    // nothing (treeReady would allocate)
+    if (!t->data || !t->model || t->root < 0 || t->data->nParts != t->model->nParts ||
+        (int)t->tp.size() != t->model->nParts) {
+        setError("treeLogLikeNoAlloc: the tree is not set up");
+        return 1;
+    }
+    for (int p = 0; p < t->nParts; p++) {
+        const TreePart &tp = t->tp[p];
+        const ModelPart *mp = t->model->parts[p];
+        const Part *dp = t->data->parts[p];
+        if (!mp || !dp || !dp->dev.valid || !tp.alloc || tp.precision != t->precision ||
+            tp.wave != pruneUsesWave(mp->dim, mp->nCat, t->precision)) {
+            setError("treeLogLikeNoAlloc: part " + std::to_string(p) +
+                     " has no device state for the tree's current precision and kernel; evaluate it once with "
+                     "p4g_treeLogLike first (that call allocates)");
+            return 1;
+        }
+    }
+    double sum = 0.0;
+    for (int p = 0; p < t->nParts; p++) {
+        double v = 0.0;
+        if (!evalPart(t, p, true, false, &v, scr)) return 1;
         sum += v;
     }
     t->logLike = sum;

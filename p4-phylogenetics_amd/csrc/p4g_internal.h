@@ -374,6 +374,10 @@ hipError_t launchPgather(const void *PT, const int *kidP, int nKidTot, int K, vo
 hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, int nStream, int precision,
                                 size_t lds, hipStream_t s);
 
+// n words of p set to v by a small kernel (usable under stream capture,
+// unlike hipMemsetAsync)
+hipError_t launchFillU32(unsigned *p, unsigned v, int n, hipStream_t s);
+
 // Host numerics (p4g_host.hip).
 int discreteGamma(double *freqK, double *rK, double alfa, double beta, int K, int median);
 

@@ -1987,7 +1987,8 @@ hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_
 {
     // lds <= WAVE_LDS_BUDGET, within the default per-workgroup limit
     const dim3 block(64 * a.K * a.nStream);
-    const bool pv = pruneWaveVectorP(precision) && a.nStream == 1;
+    // the vector-P variant needs the program-order P stream (a.PP)
+    const bool pv = pruneWaveVectorP(precision) && a.nStream == 1 && a.PP != nullptr;
     if (precision == FP32) {
         if (pv)
             hipLaunchKernelGGL((k_pruneWave<float, true, true>), dim3(nBlocks), block, lds, s, a);
@@ -2010,6 +2011,19 @@ hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, 
         hipLaunchKernelGGL((k_pruneWaveBatch<float, true>), dim3(nBlocks), block, lds, s, d, bi);
     else
         hipLaunchKernelGGL((k_pruneWaveBatch<double, false>), dim3(nBlocks), block, lds, s, d, bi);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(64) void k_fillU32(unsigned *p, unsigned v, int n)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launchFillU32(unsigned *p, unsigned v, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fillU32, dim3((n + 63) / 64), dim3(64), 0, s, p, v, n);
     return hipGetLastError();
 }
 

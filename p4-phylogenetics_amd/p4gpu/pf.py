@@ -123,6 +123,7 @@ def _declare():
     _sig("p4g_getPrecision", I, V)
     _sig("p4g_setPrams", I, V, I)
     _sig("p4g_treeLogLike", I, V, I, P_double)
+    _sig("p4g_treeLogLikeNoAlloc", I, V, P_double)
     _sig("p4g_partLogLike", I, V, V, I, I, P_double)
     _sig("p4g_calculateBigPDecks", I, V)
     _sig("p4g_calculateBigPDecksPart", I, V, I)
@@ -411,6 +412,14 @@ def p4_setPrams(cTree, pNum):
 def p4_treeLogLike(cTree, getSiteLikes):
     out = c_double(0.0)
     _ok(lib().p4g_treeLogLike(cTree, getSiteLikes, ctypes.byref(out)), "p4_treeLogLike")
+    return out.value
+
+
+def treeLogLikeNoAlloc(cTree):
+    """p4_treeLogLike(cTree, 0) with no device allocation (the tree must
+    have been evaluated once); no counterpart in p4."""
+    out = c_double(0.0)
+    _ok(lib().p4g_treeLogLikeNoAlloc(cTree, ctypes.byref(out)), "treeLogLikeNoAlloc")
     return out.value
 
 
