@@ -2,11 +2,16 @@
 
 One step = one full evaluation, the equivalent of p4's
 pf.p4_setPrams(tree, -1) + pf.p4_treeLogLike(tree, 0): discrete-gamma
-rates, Q + eigensystems, every P matrix, every partial, and the weighted
-site sum.  Workload: 64 taxa, 50,000 simulated sites, tree-heterogeneous
-DNA (3 compositions x 2 GTR matrices on nodes), G4 + pInvar 0.2, FP32
-partials with power-of-two scale exponents (the FP64 path is reported
-beside it).  Inputs are resident in GPU memory when timing starts.
+rates, Q and every P matrix, every partial, and the weighted site sum.
+Workload: 64 taxa, 50,000 simulated sites, tree-heterogeneous DNA (3
+compositions x 2 GTR matrices on nodes), G4 + pInvar 0.2.
+
+The headline is the FP64 path (the reference's precision); the FP32 path
+with power-of-two scale exponents is reported beside it with its own
+roofline.  Inputs are resident in GPU memory when timing starts.  The CPU
+baseline is the reference's own Pf code (compiled from Pf/*.c into
+oracle/_ref) evaluating the same workload, all patterns, on one core and on
+N cores (N independent processes), timed before the GPU is touched.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
@@ -14,6 +19,7 @@ Prints one JSON line.
 """
 import argparse
 import json
+import multiprocessing
 import os
 import sys
 import time
@@ -22,6 +28,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "p4-phylogenetics_amd"))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "traffic_het64.json")
 
 
 def algorithmic_bytes(tree, nPat, elt):
@@ -36,6 +43,23 @@ def algorithmic_bytes(tree, nPat, elt):
     per_pattern = 2 * eInt * S * K * elt + nLeaves * 1 + 4 + (4 if mp.pInvar.val > 0 else 0)
     nBranches = len(tree.nodes) - 1
     return nPat * per_pattern + nBranches * K * S * S * elt, per_pattern
+
+
+def kernel_name(prec):
+    """The pruning kernel the tuning applies for this precision."""
+    from p4gpu import pf
+    G = None
+    try:
+        with open(pf.TUNING_PATH) as fh:
+            G = int(json.load(fh)[prec]["opsPerGroup"])
+    except (OSError, ValueError, KeyError):
+        pass
+    t = "float, true" if prec == "fp32" else "double, false"
+    if G is None or G == -3:
+        return "k_pruneLean<%s>" % t
+    if G <= 0:
+        return "k_pruneWave<%s, %s>" % (t, "true" if G == 0 else "false")
+    return "k_prune<4, %s>" % t
 
 
 def run_gpu(tree, steps, warmup, pf):
@@ -55,37 +79,95 @@ def run_gpu(tree, steps, warmup, pf):
     return (t1 - t0) / steps, sum(kus) / len(kus), lnl
 
 
-def cpu_baseline(tree, target_seconds=12.0):
-    """The reference's own Pf CPU path (its P, eigen, gamma and
-    conditional-likelihood functions compiled from Pf/*.c into
-    oracle/_ref/libp4ref.so, driven by oracle/ref_driver.c), one core, on a
-    bounded sample of the same workload's patterns.  Without that library,
-    the oracle (scalar FP64 restatement of the same loop nest)."""
+# ------------------------------------------------------------ CPU baseline
+def _cpu_worker(flat_kwargs, seconds, out):
     sys.path.insert(0, os.path.join(HERE, "oracle"))
-    import numpy
     import oracle
-    pats, cnts, _, _, _ = tree.data.parts[0].patterns()
-    nPat = pats.shape[1]
-    sample = min(nPat, 2000)
-    f = oracle.flat_from_tree(tree, 0, patterns=numpy.ascontiguousarray(pats[:, :sample]),
-                              counts=cnts[:sample])
-    if oracle.ref() is not None:
-        kind, run = "reference", lambda: oracle.ref_part_loglike(f)
-    else:
-        kind, run = "port", lambda: oracle.part_loglike(f)
-    run()  # warm-up
+    f = oracle.Flat(**flat_kwargs)
+    run = (lambda: oracle.ref_part_loglike(f)) if oracle.ref() is not None else (lambda: oracle.part_loglike(f))
+    run()
     n, t0 = 0, time.perf_counter()
     while True:
         run()
         n += 1
         el = time.perf_counter() - t0
-        if el > target_seconds:
+        if el > seconds:
             break
-    per_eval_sample = el / n
-    per_eval_full = per_eval_sample * nPat / sample
-    return dict(value=1.0 / per_eval_full, unit="evals/s", cores=1, kind=kind,
-                sample="%d of %d patterns, %d evaluations in %.1f s (P, eigen, gamma and pruning each time); "
-                       "scaled to all patterns" % (sample, nPat, n, el))
+    out.put(n / el)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(tree, seconds=10.0):
+    """The reference's own Pf CPU path (its P, eigen, gamma and
+    conditional-likelihood functions compiled from Pf/*.c into
+    oracle/_ref/libp4ref.so, driven by oracle/ref_driver.c) on the whole
+    workload (all patterns), one core; then N cores running independent
+    evaluations in N processes.  Without that library, the oracle (FP64
+    restatement of the same loop nest)."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle
+    f = oracle.flat_from_tree(tree, 0)
+    kind = "reference" if oracle.ref() is not None else "port"
+    ctx = multiprocessing.get_context("spawn")
+    args = f.kwargs
+    q = ctx.Queue()
+    wait = 3 * seconds + 120
+    p = ctx.Process(target=_cpu_worker, args=(args, seconds, q))
+    p.start()
+    one = q.get(timeout=wait)
+    p.join()
+    try:
+        nCores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nCores = os.cpu_count() or 1
+    nCores = max(1, min(nCores, 16))   # the GPU box's CPU share per GPU
+    procs = [ctx.Process(target=_cpu_worker, args=(args, seconds, q)) for _ in range(nCores)]
+    for pr in procs:
+        pr.start()
+    rates = [q.get(timeout=wait) for _ in procs]
+    for pr in procs:
+        pr.join()
+    nPat = int(f.patterns.shape[1])
+    return dict(value=one, unit="evals/s", cores=1, kind=kind,
+                sample="all %d patterns, FP64, repeated full evaluations (P, eigen, gamma and pruning each time) "
+                       "for %.0f s" % (nPat, seconds),
+                n_core={"value": sum(rates), "unit": "evals/s", "cores": nCores,
+                        "per_process": rates},
+                cpu_model=cpu_model())
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from profiles/traffic_het64.json
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE of the same kernel on this workload,
+    with the commit they were measured at); None if that file does not hold
+    this kernel."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            d = json.load(fh)
+    except (OSError, ValueError):
+        return None, None
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None, None
+    return k.get("hbm_bytes_per_launch"), d.get("commit")
+
+
+def roofline(kernel, algo, kus):
+    achieved = algo / (kus * 1e-6) / 1e9
+    traffic, commit = traffic_for(kernel)
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_measured_at": commit,
+            "kernel": kernel, "kernel_us": kus, "algorithmic_bytes_per_launch": algo}
 
 
 def main():
@@ -95,6 +177,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="seconds of CPU work per baseline leg (1 core, then N cores)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (site patterns would be "
@@ -103,61 +187,48 @@ def main():
 
     from p4gpu import pf, synth
 
-    tree, data = synth.make_workload("het64", seed=args.seed, precision="fp32")
-    tree.calcLogLike(verbose=0)
+    tree, data = synth.make_workload("het64", seed=args.seed, precision="fp64")
     nPat = data.parts[0].nPatterns
-    sec32, kus32, lnl32 = run_gpu(tree, args.steps, args.warmup, pf)
+    cpu = None if args.no_cpu else cpu_baseline(tree, args.cpu_seconds)   # before the GPU is touched
 
-    tree.setPrecision("fp64")
     tree.calcLogLike(verbose=0)
-    sec64, kus64, lnl64 = run_gpu(tree, max(args.steps // 2, 1), max(args.warmup // 2, 1), pf)
+    sec64, kus64, lnl64 = run_gpu(tree, args.steps, args.warmup, pf)
+    tree.setPrecision("fp32")
+    tree.calcLogLike(verbose=0)
+    sec32, kus32, lnl32 = run_gpu(tree, args.steps, args.warmup, pf)
     rel32 = abs(lnl32 - lnl64) / abs(lnl64)
 
+    algo64, perpat64 = algorithmic_bytes(tree, nPat, 8)
     algo32, perpat32 = algorithmic_bytes(tree, nPat, 4)
-    algo64, _ = algorithmic_bytes(tree, nPat, 8)
-    achieved = algo32 / (kus32 * 1e-6) / 1e9
-
-    kname = "k_prune<4,float,true>"
-    try:
-        with open(pf.TUNING_PATH) as fh:
-            if int(json.load(fh)["fp32"]["opsPerGroup"]) <= 0:
-                kname = "k_pruneWave<float,true>"
-    except (OSError, ValueError, KeyError):
-        kname = "k_pruneWave<float,true>"
-    traffic = None
-    tf = os.path.join(HERE, "profiles", "traffic_het64_fp32.json")
-    if os.path.exists(tf):
-        with open(tf) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    rl64 = roofline(kernel_name("fp64"), algo64, kus64)
+    rl64["bytes_per_pattern"] = perpat64
+    rl32 = roofline(kernel_name("fp32"), algo32, kus32)
+    rl32["bytes_per_pattern"] = perpat32
 
     out = {
         "metric": "log-like evals/s (50k-site DNA, 1 GPU)",
-        "value": 1.0 / sec32,
+        "value": 1.0 / sec64,
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sec32 * 1e3,
+        "ms_per_step": sec64 * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic (seeded simulation under the model)",
         "config": {"workload": "het64: 64 taxa, 50,000 sites (%d patterns), 3 comps x 2 GTR "
-                               "node-assigned, G4 + pInvar 0.2, FP32 partials + scale exponents" % nPat,
+                               "node-assigned, G4 + pInvar 0.2, FP64 partials" % nPat,
                    "seed": args.seed, "patterns": nPat},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kname, "kernel_us": kus32,
-                     "algorithmic_bytes_per_launch": algo32, "bytes_per_pattern": perpat32},
-        "lnL_fp32": lnl32,
+        "roofline": rl64,
         "lnL_fp64": lnl64,
-        "lnL_rel_err_fp32_vs_fp64": rel32,
-        "fp64_path": {"value": 1.0 / sec64, "unit": "evals/s", "kernel_us": kus64,
-                      "achieved_GBs": algo64 / (kus64 * 1e-6) / 1e9},
+        "fp32_path": {"value": 1.0 / sec32, "unit": "evals/s", "ms_per_step": sec32 * 1e3,
+                      "dtype": "f32 partials + int exponents", "roofline": rl32, "lnL": lnl32,
+                      "lnL_rel_err_vs_fp64": rel32, "steps": args.steps},
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(tree)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
 

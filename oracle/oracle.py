@@ -169,6 +169,7 @@ class Flat(object):
     """Arrays describing one partition of a tree, kept alive together."""
 
     def __init__(self, **kw):
+        self.kwargs = dict(kw)   # picklable: Flat(**f.kwargs) rebuilds it in another process
         self.__dict__.update(kw)
         self._build()
 

@@ -392,7 +392,7 @@ static bool allocTreePart(Tree *t, int pNum)
         tp.offPack = tp.offEigList + up16(sizeof(int) * tp.nEig);
         tp.offPi = tp.offPack + up16(tp.packBytes);
         tp.offProg = tp.offPi + up16(sizeof(double) * MAX_DIM);
-        tp.progCapWords = 24 * t->nNodes + 64;
+        tp.progCapWords = 48 * t->nNodes + 128;
         tp.arenaBytes = tp.offProg + sizeof(int) * tp.progCapWords;
         if (!hostAlloc(&tp.hArena, tp.arenaBytes, "hipHostMalloc staging arena") ||
             !devAlloc(&tp.dArena, tp.arenaBytes, "hipMalloc staging arena") ||
@@ -803,6 +803,7 @@ static uint64_t programKey(const Tree *t, int pNum, bool full)
     mix(t->precision);
     mix(pruneWaveStreams(t->precision));
     mix(pruneWaveVectorP(t->precision));
+    mix(pruneWaveLean(t->precision));
     mix(pruneGroupWidthFor(tp.dim, t->precision));
     mix(tp.layoutGen);
     mix(t->root);
@@ -986,8 +987,18 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     // 1's root children from junction slots in stream 0's slot file.
     const int rootOp = order.back();
     const int nStream = (pruneWaveStreams(t->precision) && tp.K * 2 <= WAVE_MAX_K && nOps > 3) ? 2 : 1;
+    // The lean kernel stages the P^T blocks of every tip child of the
+    // program in LDS when they fit beside a few slots; otherwise its ops
+    // with tip children take the generic path.
+    int nTipKid = 0;
+    if (pruneWaveLean(t->precision) && tipLds && nStream == 1) {
+        for (int i = 0; i < nOps; i++)
+            for (int c = 0; c < ops[i].nKids; c++)
+                if (kids[ops[i].kidStart + c].kind == 0) nTipKid++;
+        if (pruneWaveMaxSlots(tp.K, tipLds, data->nTax, nTipKid, t->precision) < 4) nTipKid = 0;
+    }
     // LDS budget shared by all slot files of the workgroup
-    const int maxSlotsS = pruneWaveMaxSlots(tp.K * nStream, tipLds, data->nTax, t->precision);
+    const int maxSlotsS = pruneWaveMaxSlots(tp.K * nStream, tipLds, data->nTax, nTipKid, t->precision);
     std::vector<int> top(nOps, -1);
     for (int q = nOps - 1; q >= 0; q--) {
         const int i = order[q];
@@ -1088,7 +1099,10 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     for (int i = 0; i < nOps; i++) nSide += std::max(0, ops[i].nKids - 3);
     int nKidTot = 0;
     for (int i = 0; i < nOps; i++) nKidTot += ops[i].nKids;
-    const size_t words = (size_t)16 * nOps + 4 * (size_t)nSide + 3 * (size_t)nPre + (size_t)nKidTot;
+    const size_t wordsW = (size_t)16 * nOps + 4 * (size_t)nSide + 3 * (size_t)nPre + (size_t)nKidTot;
+    const size_t popsOff = (wordsW + 15) & ~(size_t)15;   // LeanOps, 64-byte aligned
+    const size_t tipPnOff = popsOff + (size_t)16 * nOps;
+    const size_t words = tipPnOff + (size_t)nTipKid;
     if ((int)words > tp.progCapWords) {
         setError("internal: node program larger than its staging area");
         return false;
@@ -1153,6 +1167,65 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
             wp[3 * np + 2] = cleans[j].st;
             np++;
         }
+    // the lean kernel's records: precomputed offsets for binary ops whose
+    // children are staged tips or LDS slots; the staged tips' P^T nodes
+    {
+        const size_t elt = t->precision == FP32 ? sizeof(float) : sizeof(double);
+        const size_t blkB = 16 * elt;
+        LeanOp *lo = reinterpret_cast<LeanOp *>(tp.hProg + popsOff);
+        int *tipPn = tp.hProg + tipPnOff;
+        int nt = 0;
+        for (int q = 0; q < nOps; q++) {
+            const WaveOp &w = wo[q];
+            LeanOp r{};
+            r.kind = PK_GEN;
+            r.gen = q;
+            r.ld = -1;
+            const bool tipsOk = nTipKid > 0;
+            const bool fast = w.nKids == 2 && !(w.flags & OP_ROOT) && w.ldsDst >= -1 &&
+                              ((w.kind[0] == WK_TIP && tipsOk) || w.kind[0] == WK_LDS) &&
+                              ((w.kind[1] == WK_TIP && tipsOk) || w.kind[1] == WK_LDS);
+            if (fast) {
+                int c0 = 0, c1 = 1;
+                if (w.kind[0] != WK_TIP && w.kind[1] == WK_TIP) std::swap(c0, c1);   // tips first
+                const int nTip = (w.kind[0] == WK_TIP) + (w.kind[1] == WK_TIP);
+                r.kind = nTip == 2 ? PK_TT : nTip == 1 ? PK_TL : PK_LL;
+                const int cs[2] = {c0, c1};
+                int *aa[2] = {&r.a0, &r.a1}, *tt[2] = {&r.t0, &r.t1}, *pp[2] = {&r.p0, &r.p1};
+                for (int u = 0; u < 2; u++) {
+                    const int c = cs[u];
+                    if (w.kind[c] == WK_TIP) {
+                        if (nt >= nTipKid) {
+                            setError("internal: staged tip count");
+                            return false;
+                        }
+                        *aa[u] = w.idx[c] * 64;
+                        *tt[u] = (int)(nt * blkB);
+                        tipPn[nt++] = w.pnode[c];
+                    } else {
+                        *aa[u] = w.idx[c];
+                        *tt[u] = w.idx[c] * 64;
+                        const size_t pb = (size_t)w.pnode[c] * tp.K * blkB;
+                        if (pb > (size_t)INT32_MAX) {
+                            setError("internal: P table too large for 32-bit offsets");
+                            return false;
+                        }
+                        *pp[u] = (int)pb;
+                    }
+                }
+                const size_t de = (size_t)w.dst * tp.K * tp.nPatPad;
+                r.dstLo = (int)(uint32_t)(de & 0xffffffffu);
+                r.dstHi = (int)(uint32_t)(de >> 32);
+                r.ld = w.ldsDst >= 0 ? w.ldsDst : -1;
+            }
+            lo[q] = r;
+        }
+        // tips that only generic ops read are staged too (unused, harmless)
+        for (; nt < nTipKid; nt++) tipPn[nt] = 0;
+    }
+    tp.popsOff = (int)popsOff;
+    tp.tipPnOff = (int)tipPnOff;
+    tp.nTipKid = nTipKid;
     tp.nWOps = nOps;
     tp.nWKids = nSide;
     tp.nPre = nPre;
@@ -1161,7 +1234,7 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     tp.nKidTot = nKidTot;
     tp.nStream = nStream;
     tp.nS0 = (int)lists[0].size();
-    tp.waveLds = pruneWaveLdsBytes(tp.K * nStream, nLds, tipLds, data->nTax, t->precision);
+    tp.waveLds = pruneWaveLdsBytes(tp.K * nStream, nLds, tipLds, data->nTax, nTipKid, t->precision);
     tp.progWords = (int)words;
     tp.nGroups = 0;
     tp.nKids = (int)kids.size();
@@ -1194,6 +1267,7 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     a.invMask = dp->dev.invMask;
     a.nWords = dp->dev.nWords;
     a.ambMask = dp->dev.ambMask;
+    a.nAmb = dp->dev.nAmb;
     a.rootPi = reinterpret_cast<const double *>(tp.dArena + tp.offPi);
     a.pInvar = mp->pInvar;
     a.rootLeafRow = root->isLeaf ? root->seqNum : -1;
@@ -1215,6 +1289,9 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
         a.nTax = dp->nTax;
         a.PP = tp.PP;
         a.nKidTot = tp.nKidTot;
+        a.pops = reinterpret_cast<const LeanOp *>(prog + tp.popsOff);
+        a.nTipKid = tp.nTipKid;
+        a.tipPn = prog + tp.tipPnOff;
     }
 }
 
@@ -1281,7 +1358,8 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL, 
     PruneArgs a{};
     fillPruneArgs(t, pNum, nOps, a, siteLikes);
     hipStream_t s = stream();
-    const int nBlocks = tp.wave ? tp.nPatPad / 64 : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision, nOps);
+    const int nBlocks = tp.wave ? tp.nPatPad / pruneWaveTilePatterns(t->precision)
+                                : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision, nOps);
     if (scr) {
         // [counter, 256 B][flags int[nBlocks]][sums double[nBlocks]]
         const size_t offBad = 256, offSums = offBad + (((size_t)nBlocks * sizeof(int) + 255) & ~(size_t)255);
@@ -2437,7 +2515,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     auto itemBlocks = [](const Item &it) {
         const ModelPart *mp = it.t->model->parts[it.p];
         const TreePart &tp = it.t->tp[it.p];
-        return tp.wave ? tp.nPatPad / 64
+        return tp.wave ? tp.nPatPad / pruneWaveTilePatterns(it.t->precision)
                        : tp.nPatPad / pruneBlockPatternsFor(mp->dim, mp->nCat, it.t->precision, it.nOps);
     };
     for (auto &it : items) nBlocksTot += itemBlocks(it);

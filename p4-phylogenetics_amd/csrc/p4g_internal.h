@@ -157,6 +157,26 @@ static_assert(sizeof(WaveOp) == 64, "WaveOp is one 64-byte scalar load");
 struct WaveKid {
     int kind, idx, pnode, pad;
 };
+// The lean kernel's op records (one per WaveOp, same order): binary ops
+// whose children are staged tips or LDS slots carry precomputed offsets, so
+// the kernel does no index arithmetic; every other op (the root,
+// multifurcations, children in global memory, unstaged tips) is PK_GEN and
+// runs from its WaveOp.  Tip children come first.
+enum { PK_TT = 0, PK_TL = 1, PK_LL = 2, PK_GEN = 3 };
+struct LeanOp {
+    int kind;
+    int a0, a1;          // tip: code offset row * 64; LDS child: slot index
+    int t0, t1;          // tip: byte offset of its staged P^T block in the category's region;
+                         // LDS child: exponent index slot * 64
+    int p0, p1;          // LDS child: byte offset of its P^T block, node * K * 16 * element
+    int dstLo, dstHi;    // result slot * K * nPatPad (in 4-state vectors)
+    int pad0, pad1;
+    int ld;              // LDS slot of the result, or -1
+    int pad2;
+    int gen;             // WaveOp index (PK_GEN)
+    int pad3, pad4;
+};
+static_assert(sizeof(LeanOp) == 64, "LeanOp is one 64-byte scalar load");
 constexpr int WAVE_MAX_K = 8;             // categories (waves per workgroup)
 constexpr int WAVE_MAX_LDS_SLOTS = 16;
 constexpr int WAVE_TIP_LDS_MAX_TAXA = 256;
@@ -203,6 +223,8 @@ struct TreePart {
     bool wave = false;
     int nWOps = 0, nWKids = 0, nPre = 0, nLds = 0, tipLds = 0, nKidTot = 0;
     int nStream = 1, nS0 = 0;
+    int popsOff = 0;               // LeanOp records, in ints from the program start
+    int tipPnOff = 0, nTipKid = 0; // staged tip children's P^T nodes
     // The staged program is reused while everything it depends on is
     // unchanged (tree structure, postorder, marks, kernel choice, layout).
     bool progValid = false;
@@ -321,6 +343,12 @@ struct PruneArgs {
     // P^T of every child in program order, [nKidTot][K][16] (k_pgather)
     const void *PP;
     int nKidTot;
+    // lean kernel: one LeanOp per WaveOp; the P^T nodes of the staged tip
+    // children (nTipKid, in the order of their staging)
+    const LeanOp *pops;
+    int nTipKid;
+    const int *tipPn;
+    int nAmb;                // entries of ambMask
 };
 
 // Kernel launchers (p4g_kernels.hip).
@@ -363,10 +391,12 @@ hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, in
 // kernel: partials [slot][cat][pattern][state], FP32 exponents
 // [slot][cat][pattern]; 64 patterns per workgroup of 64*K threads.
 bool pruneUsesWave(int dim, int K, int precision);
+bool pruneWaveLean(int precision);      // the lean kernel runs the wave program
+int pruneWaveTilePatterns(int precision);  // patterns per workgroup of the wave program's kernel
 bool pruneWaveVectorP(int precision);   // the wave kernel reads the k_pgather P stream
 bool pruneWaveStreams(int precision);   // the wave kernel splits the tree into two streams
-size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int precision);
-int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision);
+size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int nTipKid, int precision);
+int pruneWaveMaxSlots(int K, int tipLds, int nTax, int nTipKid, int precision);
 hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s);
 // P^T of the program's children in program order: PP[k][j] = PT[kidP[j]][k]
 hipError_t launchPgather(const void *PT, const int *kidP, int nKidTot, int K, void *PP, int precision,

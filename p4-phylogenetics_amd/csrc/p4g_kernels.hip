@@ -999,7 +999,10 @@ static int g_tuneLds[2] = {1, 1};
 static int g_wave[2] = {1, 1};       // 1: 4-state parts use k_pruneWave
 static int g_waveVecP[2] = {0, 0};   // 1: its vector-prefetched P stream
 static int g_waveStreams[2] = {0, 0};  // 1: two streams of category waves (scalar P)
+static int g_waveLean[2] = {1, 1};   // 1: the lean kernel (LeanOp records, tip P^T in LDS)
 
+// G = -3 selects the lean kernel (the default: the wave kernel's layout with
+// precomputed records and tip P^T staged in LDS);
 // G = 0 selects the wave-per-category kernel for 4-state parts with its
 // vector-prefetched P stream, G = -1 the same kernel with P by scalar
 // loads, G = -2 that with the tree split into two streams of waves; G = 1
@@ -1007,8 +1010,9 @@ static int g_waveStreams[2] = {0, 0};  // 1: two streams of category waves (scal
 void setPruneTuning(int precision, int G, int lds)
 {
     if (precision < 0 || precision > 1) return;
-    if (G == 0 || G == -1 || G == -2) {
+    if (G == 0 || G == -1 || G == -2 || G == -3) {
         g_wave[precision] = 1;
+        g_waveLean[precision] = G == -3 ? 1 : 0;
         g_waveVecP[precision] = G == 0 ? 1 : 0;
         g_waveStreams[precision] = G == -2 ? 1 : 0;
         return;
@@ -1519,6 +1523,9 @@ template <> struct Vec4T<float> { typedef float4 type; };
 template <> struct Vec4T<double> { typedef double4 type; };
 
 constexpr int WAVE_LDS_BUDGET = 48 * 1024;   // dynamic LDS per workgroup
+// s_waitcnt vmcnt(0) with the other counters left alone (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8)
+constexpr int VMCNT0 = 0x0F70;
 
 // leafVec for 4 states with P^T in registers.  A single-state code (the
 // common case) picks its row with a 2-bit select tree; gaps, '?' and
@@ -1929,6 +1936,324 @@ __global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneWaveBatch(const PruneA
     pruneWaveBody<T, SCALE, false>(a, local, a.nPatPad / 64);
 }
 
+// ------------------------------------------- lean DNA kernel (default)
+// The wave kernel's lane mapping (lane = pattern, wavefront = gamma
+// category, workgroup = the K category waves of a 64-pattern tile) and
+// program, with three changes that cut the instructions per node op:
+//  * binary ops whose children are tips or LDS slots run from LeanOp
+//    records of precomputed byte offsets (one 64-byte scalar load);
+//  * the P^T blocks of the program's tip children are staged in LDS at the
+//    start, so a tip child is one 16-byte LDS read of row `code` instead of
+//    a select tree over P held in scalar registers (16 moves + 12 selects);
+//  * an internal child's P^T comes by one 64-byte scalar load per child and
+//    feeds the 16 FMAs directly as scalar operands.
+// The root, multifurcations and children in global memory take the generic
+// path of the wave kernel.
+template <typename T, bool SCALE>
+__device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int nTiles)
+{
+    typedef typename Vec4T<T>::type V4;
+    typedef typename ConstT<T>::type cT;
+    typedef __attribute__((address_space(3))) T lT;
+    typedef __attribute__((address_space(3))) int lint;
+    typedef __attribute__((address_space(3))) const unsigned char lu8;
+    typedef __attribute__((address_space(3))) char lchar;
+    typedef __attribute__((address_space(1))) const T gT;
+    typedef __attribute__((address_space(1))) const int gint;
+    typedef __attribute__((address_space(1))) const unsigned char gu8;
+    typedef __attribute__((address_space(4))) const char cchar;
+    typedef int rec16 __attribute__((ext_vector_type(16)));
+    typedef const __attribute__((address_space(4))) rec16 crec16;
+    extern __shared__ __align__(16) unsigned char smemW[];
+    __shared__ double rootL[WAVE_MAX_K][64];
+    __shared__ int rootE[WAVE_MAX_K][64];
+    __shared__ unsigned char ambL[256];   // state masks of partial-ambiguity codes
+    const int K = a.K;
+    const int cat = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int p = tile * 64 + lane;
+    const size_t nPP = (size_t)a.nPatPad;
+    const int nL = a.nLds;
+    const int tipBytes = a.tipLds ? ((a.nTax * 64 + 15) & ~15) : 0;
+    const int blkB = 16 * (int)sizeof(T);                   // one P^T block
+    for (int j = threadIdx.x; j < a.nAmb && j < 251; j += blockDim.x) ambL[j] = (unsigned char)(a.ambMask[j] & 15u);
+    const int tipPBytes = K * a.nTipKid * blkB;
+    lchar *lds = (lchar *)smemW;
+    lu8 *tipCodes = (lu8 *)lds + lane;                      // + row * 64
+    lchar *tipP = lds + tipBytes + cat * a.nTipKid * blkB;  // + t * blkB + code * 4 * sizeof(T)
+    // slot l of this wave: slotV[l * 256 elements], exponent slotE[l * 64]
+    lT *slotV = (lT *)(lds + tipBytes + tipPBytes) + ((size_t)cat * nL * 64 + lane) * 4;
+    lint *slotE = (lint *)(lds + tipBytes + tipPBytes + (size_t)K * nL * 64 * 4 * sizeof(T)) + (size_t)cat * nL * 64 +
+                  lane;
+    gu8 *gTips = (gu8 *)(a.tips);
+    gint *gScl = (gint *)(a.scl);
+    if (a.tipLds) {
+        // the tile's tip codes, [taxon][64 patterns], 4 bytes per thread
+        const int words = a.nTax * 16;
+        for (int w = threadIdx.x; w < words; w += blockDim.x) {
+            const int r = w >> 4, q = w & 15;
+            reinterpret_cast<uint32_t *>(smemW)[w] =
+                *reinterpret_cast<const uint32_t *>(a.tips + (size_t)r * nPP + (size_t)tile * 64 + q * 4);
+        }
+        // the P^T blocks of the staged tip children, [cat][t][16], 16 bytes per thread
+        const int chunks = K * a.nTipKid * blkB / 16;
+        const int perBlk = blkB / 16;
+        for (int w = threadIdx.x; w < chunks; w += blockDim.x) {
+            const int blk = w / perBlk, part = w - blk * perBlk;
+            const int k = blk / a.nTipKid, t = blk - k * a.nTipKid;
+            const char *src = (const char *)a.PT + ((size_t)a.tipPn[t] * K + k) * blkB + part * 16;
+            *reinterpret_cast<uint4 *>(smemW + tipBytes + (size_t)blk * blkB + part * 16) =
+                *reinterpret_cast<const uint4 *>(src);
+        }
+        __syncthreads();
+    }
+    gT *clr = (gT *)(a.cl);
+    V4 *clw = reinterpret_cast<V4 *>(a.cl);
+    auto ldG = [&](size_t o) {
+        V4 x;
+        x.x = clr[4 * o];
+        x.y = clr[4 * o + 1];
+        x.z = clr[4 * o + 2];
+        x.w = clr[4 * o + 3];
+        return x;
+    };
+    auto ldS = [&](int sl) {
+        V4 x;
+        x.x = slotV[sl * 256];
+        x.y = slotV[sl * 256 + 1];
+        x.z = slotV[sl * 256 + 2];
+        x.w = slotV[sl * 256 + 3];
+        return x;
+    };
+    auto stS = [&](int sl, const V4 &x) {
+        slotV[sl * 256] = x.x;
+        slotV[sl * 256 + 1] = x.y;
+        slotV[sl * 256 + 2] = x.z;
+        slotV[sl * 256 + 3] = x.w;
+    };
+    const size_t catOff = (size_t)cat * nPP + p;   // element of slot s: s*K*nPP + catOff
+    const size_t slotStride = (size_t)K * nPP;
+    // a staged tip child: row `code` of its P^T block (Pf/p4_node.c:660-725)
+    typedef T tv4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(3))) const tv4 ltv4;
+    auto tipRow = [&](int tOff, unsigned code, T *v) {
+        if (code < 4u) {
+            const tv4 row = *(ltv4 *)(tipP + tOff + code * 4 * (int)sizeof(T));
+            v[0] = row.x;
+            v[1] = row.y;
+            v[2] = row.z;
+            v[3] = row.w;
+        } else if (code == TIP_ALL) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) v[s] = T(1);
+        } else {
+            const unsigned m = ambL[code - 4];
+            const lT *blk = (const lT *)(tipP + tOff);
+#pragma unroll
+            for (int s = 0; s < 4; s++) v[s] = T(0);
+            for (int j = 0; j < 4; j++)
+                if ((m >> j) & 1u) {
+#pragma unroll
+                    for (int s = 0; s < 4; s++) v[s] += blk[j * 4 + s];
+                }
+        }
+    };
+    cchar *PTc = (cchar *)(a.PT) + (size_t)cat * blkB;
+
+    const int *pre = a.wpre;
+    for (int i = 0; i < a.nPre; i++) {
+        const size_t o = (size_t)__builtin_amdgcn_readfirstlane(pre[3 * i]) * slotStride + catOff;
+        const int l = __builtin_amdgcn_readfirstlane(pre[3 * i + 1]);
+        stS(l, ldG(o));
+        if (SCALE) slotE[l * 64] = gScl[o];
+    }
+    // no global load is left in flight past this point or past a generic
+    // op (the fast ops issue none), so the fast path never waits on the
+    // vector-memory counter, which also counts the partials' stores
+    __builtin_amdgcn_s_waitcnt(VMCNT0);
+
+    crec16 *lops = (crec16 *)(a.pops);
+    cint_t *opsS = (cint_t *)(a.wops);
+    cint_t *sideS = (cint_t *)(a.wkids);
+    const int nW = a.nWOps;
+    double contrib = 0.0;
+    int bad = 0;
+    // FP32 rescaling by the pattern's largest entry, an exact power of two
+    auto rescale = [&](T *acc, int &e) {
+        if (SCALE) {
+            const T m = fmax(fmax(acc[0], acc[1]), fmax(acc[2], acc[3]));
+            if (m < T(SCALE_THRESH) && m > T(0)) {
+                int ex;
+                frexp(m, &ex);
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc[s] = ldexp(acc[s], -ex);
+                e -= ex;
+            }
+        }
+    };
+    // the result to HBM and, when its parent is in the program, to LDS
+    auto store = [&](const T *acc, int e, size_t dstE, int ld) {
+        V4 rv;
+        rv.x = acc[0];
+        rv.y = acc[1];
+        rv.z = acc[2];
+        rv.w = acc[3];
+        clw[dstE + catOff] = rv;
+        if (SCALE) a.scl[dstE + catOff] = e;
+        if (ld >= 0) {
+            stS(ld, rv);
+            if (SCALE) slotE[ld * 64] = e;
+        }
+    };
+    // Fast ops run in an inner loop of their own: no global load is ever in
+    // flight there, so the compiler's wait-count analysis never makes a fast
+    // op wait on the vector-memory counter, which also counts the partials'
+    // stores.  Generic ops (and the root) run between, and wait for their
+    // own loads before the fast loop resumes.
+    int i = 0;
+    while (true) {
+        rec16 rn = lops[i];   // each record is one 64-byte scalar load, one op ahead
+        for (; i < nW; i++) {
+            const rec16 r = rn;
+            const int kind = r[0];
+            if (kind == PK_GEN) break;
+            if (i + 1 < nW) rn = lops[i + 1];
+            T u[4], w[4], acc[4];
+            int e = 0;
+            if (kind == PK_LL) {
+                const V4 x = ldS(r[1]), y = ldS(r[2]);
+                if (SCALE) e = slotE[r[3]] + slotE[r[4]];
+                cT *P0 = (cT *)(PTc + r[5]);
+                cT *P1 = (cT *)(PTc + r[6]);
+                T Pa[16], Pb[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    Pa[j] = P0[j];
+                    Pb[j] = P1[j];
+                }
+                const T xv[4] = {x.x, x.y, x.z, x.w}, yv[4] = {y.x, y.y, y.z, y.w};
+                internalVec<4, T>(Pa, xv, 4, u);
+                internalVec<4, T>(Pb, yv, 4, w);
+            } else {
+                tipRow(r[3], (unsigned)tipCodes[r[1]], u);
+                if (kind == PK_TT) {
+                    tipRow(r[4], (unsigned)tipCodes[r[2]], w);
+                } else {
+                    const V4 y = ldS(r[2]);
+                    if (SCALE) e = slotE[r[4]];
+                    cT *P1 = (cT *)(PTc + r[6]);
+                    T Pb[16];
+#pragma unroll
+                    for (int j = 0; j < 16; j++) Pb[j] = P1[j];
+                    const T yv[4] = {y.x, y.y, y.z, y.w};
+                    internalVec<4, T>(Pb, yv, 4, w);
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) acc[s] = u[s] * w[s];
+            rescale(acc, e);
+            store(acc, e, (size_t)(unsigned)r[7] | ((size_t)(unsigned)r[8] << 32), r[11]);
+        }
+        if (i >= nW) break;
+        // ---- a generic op: any children, from anywhere; maybe the root
+        cint_t *op = opsS + 16 * i;
+        const int nk = op[3];
+        const int kidStart = op[4];
+        const bool isRoot = (op[2] & OP_ROOT) != 0;
+        T acc[4];
+        int e = 0;
+        for (int c = 0; c < nk; c++) {
+            int ck, idx, pn;
+            if (c < 3) {
+                ck = op[5 + c];
+                idx = op[8 + c];
+                pn = op[11 + c];
+            } else {
+                cint_t *kr = sideS + 4 * (kidStart + c - 3);
+                ck = kr[0];
+                idx = kr[1];
+                pn = kr[2];
+            }
+            cT *Pk = (cT *)(PTc + (size_t)pn * K * blkB);
+            T v[4];
+            T Pl[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) Pl[j] = Pk[j];
+            if (ck == WK_TIP) {
+                const unsigned code = a.tipLds ? (unsigned)tipCodes[idx * 64] : (unsigned)gTips[(size_t)idx * nPP + p];
+                leafVec4<T>(Pl, code, a.ambMask, v);
+            } else {
+                V4 x;
+                int ex = 0;
+                if (ck == WK_LDS) {
+                    x = ldS(idx);
+                    if (SCALE) ex = slotE[idx * 64];
+                } else {
+                    const size_t o = (size_t)idx * slotStride + catOff;
+                    x = ldG(o);
+                    if (SCALE) ex = gScl[o];
+                }
+                const T xv[4] = {x.x, x.y, x.z, x.w};
+                internalVec<4, T>(Pl, xv, 4, v);
+                e += ex;
+            }
+            if (c == 0) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc[s] = v[s];
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; s++) acc[s] *= v[s];
+            }
+        }
+        rescale(acc, e);
+        if (!isRoot) {
+            store(acc, e, (size_t)op[0] * slotStride, op[1]);
+        } else {
+            // ---- root: category-summed site likelihood (Pf/p4_tree.c:1029-1378)
+            const double *pi = a.rootPi;
+            double Lk = 0.0;
+            if (a.rootLeafRow >= 0) {
+                const unsigned code = a.tips[(size_t)a.rootLeafRow * nPP + p];
+                const uint64_t m = code < 4u ? (1ull << code) : code == TIP_ALL ? ~0ull : a.ambMask[code - 4];
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+                    if ((m >> s) & 1ull) Lk += pi[s] * (double)acc[s];
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; s++) Lk += pi[s] * (double)acc[s];
+            }
+            rootL[cat][lane] = Lk;
+            rootE[cat][lane] = e;
+            __syncthreads();
+            if (cat == 0) {
+                int mn = rootE[0][lane];
+                for (int k = 1; k < K; k++) mn = min(mn, rootE[k][lane]);
+                double L = 0.0;
+                for (int k = 0; k < K; k++) L += SCALE ? ldexp(rootL[k][lane], mn - rootE[k][lane]) : rootL[k][lane];
+                siteLogLike<SCALE>(a, L, mn, p, K, pi, contrib, bad);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(VMCNT0);
+        i++;
+    }
+    if (cat == 0) finishSiteSumWave(a, contrib, bad, tile, nTiles);
+}
+
+template <typename T, bool SCALE>
+__global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneLean(PruneArgs a)
+{
+    pruneLeanBody<T, SCALE>(a, blockIdx.x, gridDim.x);
+}
+
+template <typename T, bool SCALE>
+__global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneLeanBatch(const PruneArgs *args, const int *blockItem)
+{
+    const int item = blockItem[2 * blockIdx.x];
+    const int local = blockItem[2 * blockIdx.x + 1];
+    const PruneArgs a = args[item];
+    pruneLeanBody<T, SCALE>(a, local, a.nPatPad / 64);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_pgather(const T *PT, const int *kidP, int nKidTot, int K, T *PP)
 {
@@ -1954,13 +2279,26 @@ hipError_t launchPgather(const void *PT, const int *kidP, int nKidTot, int K, vo
     return hipGetLastError();
 }
 
-bool pruneWaveVectorP(int precision) { return g_waveVecP[precision & 1] != 0; }
-bool pruneWaveStreams(int precision) { return g_waveStreams[precision & 1] != 0 && !g_waveVecP[precision & 1]; }
+bool pruneWaveLean(int precision) { return g_waveLean[precision & 1] != 0; }
+bool pruneWaveVectorP(int precision) { return g_waveVecP[precision & 1] != 0 && !g_waveLean[precision & 1]; }
+bool pruneWaveStreams(int precision)
+{
+    return g_waveStreams[precision & 1] != 0 && !g_waveVecP[precision & 1] && !g_waveLean[precision & 1];
+}
+int pruneWaveTilePatterns(int precision) { (void)precision; return 64; }
 
 bool pruneUsesWave(int dim, int K, int precision)
 {
     return dim == 4 && K >= 1 && K <= WAVE_MAX_K && g_wave[precision & 1];
 }
+
+// dynamic LDS of the lean kernel's workgroup (plus 6 KB static for the root
+// combine; gfx950 allows 160 KB per workgroup).  A het64-sized tile needs
+// ~40 KB in FP32, so four workgroups share a CU.  In FP64 its tip blocks
+// alone take 32 KB: within this budget they are not staged (the generic
+// path), and a larger budget (two workgroups per CU) measured slower still.
+constexpr int LEAN_LDS_BUDGET32 = 56 * 1024;
+constexpr int LEAN_LDS_BUDGET64 = 56 * 1024;
 
 static size_t waveTipBytes(int tipLds, int nTax) { return tipLds ? (((size_t)nTax * 64 + 15) & ~(size_t)15) : 0; }
 
@@ -1970,16 +2308,22 @@ static size_t waveSlotBytes(int K, int precision)
     return (size_t)K * 64 * (4 * elt + (precision == FP32 ? sizeof(int) : 0));
 }
 
-size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int precision)
+// LDS of a workgroup: tip codes, the lean kernel's staged tip P^T blocks
+// (nTipKid per category; 0 for the wave kernel), the K slot files
+size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int nTipKid, int precision)
 {
-    return waveTipBytes(tipLds, nTax) + (size_t)nLds * waveSlotBytes(K, precision);
+    const size_t elt = precision == FP32 ? sizeof(float) : sizeof(double);
+    return waveTipBytes(tipLds, nTax) + (size_t)K * nTipKid * 16 * elt + (size_t)nLds * waveSlotBytes(K, precision);
 }
 
-int pruneWaveMaxSlots(int K, int tipLds, int nTax, int precision)
+int pruneWaveMaxSlots(int K, int tipLds, int nTax, int nTipKid, int precision)
 {
-    const size_t tb = waveTipBytes(tipLds, nTax);
-    if (tb >= (size_t)WAVE_LDS_BUDGET) return 0;
-    const size_t n = ((size_t)WAVE_LDS_BUDGET - tb) / waveSlotBytes(K, precision);
+    const size_t fixed = pruneWaveLdsBytes(K, 0, tipLds, nTax, nTipKid, precision);
+    const size_t budget = !pruneWaveLean(precision) ? WAVE_LDS_BUDGET
+                          : precision == FP32          ? LEAN_LDS_BUDGET32
+                                                       : LEAN_LDS_BUDGET64;
+    if (fixed >= budget) return 0;
+    const size_t n = (budget - fixed) / waveSlotBytes(K, precision);
     return (int)std::min<size_t>(n, WAVE_MAX_LDS_SLOTS);
 }
 
@@ -1987,6 +2331,13 @@ hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_
 {
     // lds <= WAVE_LDS_BUDGET, within the default per-workgroup limit
     const dim3 block(64 * a.K * a.nStream);
+    if (pruneWaveLean(precision)) {
+        if (precision == FP32)
+            hipLaunchKernelGGL((k_pruneLean<float, true>), dim3(nBlocks), block, lds, s, a);
+        else
+            hipLaunchKernelGGL((k_pruneLean<double, false>), dim3(nBlocks), block, lds, s, a);
+        return hipGetLastError();
+    }
     // the vector-P variant needs the program-order P stream (a.PP)
     const bool pv = pruneWaveVectorP(precision) && a.nStream == 1 && a.PP != nullptr;
     if (precision == FP32) {
@@ -2007,6 +2358,13 @@ hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, 
                                 size_t lds, hipStream_t s)
 {
     const dim3 block(64 * K * nStream);
+    if (pruneWaveLean(precision)) {
+        if (precision == FP32)
+            hipLaunchKernelGGL((k_pruneLeanBatch<float, true>), dim3(nBlocks), block, lds, s, d, bi);
+        else
+            hipLaunchKernelGGL((k_pruneLeanBatch<double, false>), dim3(nBlocks), block, lds, s, d, bi);
+        return hipGetLastError();
+    }
     if (precision == FP32)
         hipLaunchKernelGGL((k_pruneWaveBatch<float, true>), dim3(nBlocks), block, lds, s, d, bi);
     else

@@ -1,11 +1,13 @@
 """Autotuner for the 4-state pruning kernel.
 
-Times the wave-per-category kernel and every (ops per group, P staged in
-LDS) variant of the grouped kernel on a workload, for FP32 and FP64
-partials, and writes the fastest to p4gpu/tuning.json, which the library
-applies when it is loaded (opsPerGroup 0 / -1 / -2 = the wave kernel with its
-P^T read by vector / scalar loads / scalar loads with the tree split into two
-streams of waves).  The grouped
+Times the lean kernel (the wave kernel's layout with precomputed op records
+and tip P^T staged in LDS), the wave-per-category kernel
+and every (ops per group, P staged in LDS) variant of the grouped kernel on
+a workload, for FP32 and FP64 partials, and writes the fastest to
+p4gpu/tuning.json, which the library applies when it is loaded
+(opsPerGroup -3 = the lean kernel; 0 / -1 / -2 = the wave kernel with its
+P^T read by vector / scalar loads / scalar loads with the tree split into
+two streams of waves).  The grouped
 variants compute identical results; the wave kernel keeps FP32 scale
 exponents per category, which changes FP32 results only at rounding level.
 
@@ -40,7 +42,8 @@ def autotune(workload="het64", reps=30, seed=1, write=True, verbose=True):
         tree, _ = synth.make_workload(workload, seed=seed, precision=prec)
         tree.calcLogLike(verbose=0)
         res = {}
-        for G, label in ((-1, "wave, scalar P"), (-2, "wave, scalar P, 2 streams"), (0, "wave, vector P")):
+        for G, label in ((-3, "lean"), (-1, "wave, scalar P"), (-2, "wave, scalar P, 2 streams"),
+                         (0, "wave, vector P")):
             pf.setPruneTuning(prec == "fp32", G, 1)
             res[(G, 1)] = time_variant(tree, reps)
             if verbose:
@@ -54,7 +57,8 @@ def autotune(workload="het64", reps=30, seed=1, write=True, verbose=True):
         (G, lds), best = min(res.items(), key=lambda kv: kv[1])
         out[prec] = {"opsPerGroup": G, "stagePInLds": lds, "kernel_us": best,
                      "workload": workload,
-                     "all": {({0: "wave_vectorP", -1: "wave_scalarP", -2: "wave_scalarP_2streams"}.get(k[0]) or "G%d_lds%d" % k): v
+                     "all": {({0: "wave_vectorP", -1: "wave_scalarP", -2: "wave_scalarP_2streams",
+                                  -3: "lean"}.get(k[0]) or "G%d_lds%d" % k): v
                              for k, v in res.items()}}
         pf.setPruneTuning(prec == "fp32", G, lds)
     if write:

@@ -99,11 +99,12 @@ def test_prior_only_run_recovers_branch_length_prior():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("G", [-2, 0, 4])
+@pytest.mark.parametrize("G", [-3, -1, -2, 0, 4])
 def test_incremental_equals_full_for_every_kernel_variant(G):
-    """Dirty-path evaluations (preloaded clean children, junction slots of
-    the two-stream variant, the program-order P stream, the grouped kernel)
-    equal a full recomputation, generation after generation."""
+    """Dirty-path evaluations (preloaded clean children, the pair kernel's
+    generic and fast ops, junction slots of the two-stream variant, the
+    program-order P stream, the grouped kernel) equal a full recomputation,
+    generation after generation."""
     tree, _ = synth.make_workload("het64", seed=31, nSites=1200, precision="fp32")
     try:
         pf.setPruneTuning(True, G, 1)
@@ -116,5 +117,5 @@ def test_incremental_equals_full_for_every_kernel_variant(G):
                 v = full.calcLogLike(verbose=0)
                 assert abs(v - ch.curTree.logLike) <= 1e-7 * abs(v), (G, g, v, ch.curTree.logLike)
     finally:
-        pf.setPruneTuning(True, -1, 1)   # the library default, then the tuned choice
+        pf.setPruneTuning(True, -3, 1)   # the library default, then the tuned choice
         pf._applyTuning()
