@@ -16,6 +16,7 @@ Differences that keep the same stationary distribution but suit the GPU:
 Proposals: 'brLen' (multiplier with log reflection, p4/chain.py:3091-3141),
 'allBrLens' (sliding window, p4/chain.py:3143-3196), 'compDir'
 (Dirichlet centred on the current comp, p4/chain.py:1798-1866),
+'allCompsDir' (every comp of a part at once, p4/chain.py:2259-2289),
 'rMatrixDir' (same for the exchangeabilities), 'gdasrv' (multiplier on the
 gamma shape), 'pInvar' (sliding window), 'nni' (nearest-neighbour
 interchange, a symmetric topology move), and optionally 'learned' (see
@@ -69,6 +70,37 @@ class Proposal(object):
 def _dirichlet_logpdf(x, alpha):
     return (math.lgamma(alpha.sum()) - sum(math.lgamma(a) for a in alpha) +
             float(((alpha - 1.0) * numpy.log(x)).sum()))
+
+
+def heatedLogRatio(beta, lnlProp, lnlCur, logPriorRatio, logProposalRatio, samplePrior=False):
+    """Log Metropolis-Hastings ratio of a move in a chain of heat
+    beta = 1 / (1 + T): Metropolis-coupled MCMC powers the likelihood ratio
+    and the prior ratio by beta, not the proposal ratio (p4/chain.py:1060-1122,
+    1129).  samplePrior leaves the likelihood ratio out (the commented
+    logLikeRatio = 0.0 of p4/chain.py:1065-1066).  A proposed lnL at the
+    -1e99 sentinel (a site likelihood <= 0) is a rejection."""
+    if samplePrior:
+        return beta * logPriorRatio + logProposalRatio
+    if lnlProp <= -1.0e99 * 0.5:
+        return -1.0e300
+    return beta * (lnlProp - lnlCur) + beta * logPriorRatio + logProposalRatio
+
+
+def acceptProbability(logR):
+    """p4/chain.py:1310-1320: 0 below -100, 1 at or above 0, else exp."""
+    if logR < -100.0:
+        return 0.0
+    if logR >= 0.0:
+        return 1.0
+    return math.exp(logR)
+
+
+def swapLogRatio(beta1, beta2, lnl1, lnl2):
+    """Log ratio of exchanging the states of two chains of heats beta1 and
+    beta2 with current log likelihoods lnl1 and lnl2, as p4/mcmc.py:3315-3322
+    computes it: likelihoods only (the reference leaves the heated priors out
+    of the swap, which is exact when the prior is flat)."""
+    return beta1 * lnl2 + beta2 * lnl1 - beta1 * lnl1 - beta2 * lnl2
 
 
 class Chain(object):
@@ -153,32 +185,62 @@ class Chain(object):
         for p in range(t.model.nParts):
             self._markAll(t, p)
 
+    def _dirichletMove(self, cur, a, lo):
+        """A draw from Dirichlet(a * cur) (p4/chain.py:1798-1866) and the log
+        Hastings ratio log q(cur | new) - log q(new | cur), or None when a
+        component falls below lo.  The reference nudges such components up
+        and keeps the Dirichlet densities, which are then not the density of
+        what was proposed; here the draw is rejected instead (the prior is
+        zero there), which keeps the move exact."""
+        new = self.mcmc.nprng.dirichlet(a * cur)
+        if new.min() < lo:
+            return None, 0.0
+        new = new / new.sum()
+        return new, _dirichlet_logpdf(cur, a * new) - _dirichlet_logpdf(new, a * cur)
+
+    def _reject(self):
+        self.logProposalRatio = -1.0e300
+
     def _propose_compDir(self, prop):
         t = self.propTree
         mp = t.model.parts[prop.pNum]
         c = mp.comps[self.rng.randrange(mp.nComps)]
-        a = prop.tuning[self.tempNum]
-        cur = c.val.copy()
-        new = self.mcmc.nprng.dirichlet(a * cur)
-        while new.min() < var.PIVEC_MIN:
-            small = new < var.PIVEC_MIN
-            new[small] += (1.0 + self.mcmc.nprng.random(small.sum())) * var.PIVEC_MIN
-            new = new / new.sum()
-        new[-1] = 1.0 - new[:-1].sum()
-        self.logProposalRatio = _dirichlet_logpdf(cur, a * new) - _dirichlet_logpdf(new, a * cur)
+        new, lr = self._dirichletMove(c.val.copy(), prop.tuning[self.tempNum], var.PIVEC_MIN)
+        if new is None:
+            self._reject()
+            return
+        self.logProposalRatio = lr
         c.val[:] = new
+        self._modelChanged(prop.pNum)
+
+    def _propose_allCompsDir(self, prop):
+        """Every comp of the part in one move, each from a Dirichlet centred
+        on its current value; the log proposal ratios add
+        (p4/chain.py:2259-2289)."""
+        t = self.propTree
+        mp = t.model.parts[prop.pNum]
+        news, lr = [], 0.0
+        for c in mp.comps:
+            new, l1 = self._dirichletMove(c.val.copy(), prop.tuning[self.tempNum], var.PIVEC_MIN)
+            if new is None:
+                self._reject()
+                return
+            news.append(new)
+            lr += l1
+        self.logProposalRatio = lr
+        for c, new in zip(mp.comps, news):
+            c.val[:] = new
         self._modelChanged(prop.pNum)
 
     def _propose_rMatrixDir(self, prop):
         t = self.propTree
         mp = t.model.parts[prop.pNum]
         r = mp.rMatrices[self.rng.randrange(mp.nRMatrices)]
-        a = prop.tuning[self.tempNum]
-        cur = r.val / r.val.sum()
-        new = self.mcmc.nprng.dirichlet(a * cur)
-        new = numpy.maximum(new, var.RATE_MIN)
-        new = new / new.sum()
-        self.logProposalRatio = _dirichlet_logpdf(cur, a * new) - _dirichlet_logpdf(new, a * cur)
+        new, lr = self._dirichletMove(r.val / r.val.sum(), prop.tuning[self.tempNum], var.RATE_MIN)
+        if new is None:
+            self._reject()
+            return
+        self.logProposalRatio = lr
         r.val[:] = new
         mp.setCStuff(t.model)
         self._modelChanged(prop.pNum, pushModel=False)
@@ -299,8 +361,8 @@ def _swapSubtrees(x, u, s, p):
 class McmcTunings(object):
     def __init__(self):
         self.default = {'brLen': 2.0 * math.log(2.0), 'allBrLens': 2.0 * math.log(1.02),
-                        'compDir': 100.0, 'rMatrixDir': 200.0, 'gdasrv': 2.0 * math.log(1.5),
-                        'pInvar': 0.5, 'nni': 1.0, 'learned': 1.0}
+                        'compDir': 100.0, 'allCompsDir': 500.0, 'rMatrixDir': 200.0,
+                        'gdasrv': 2.0 * math.log(1.5), 'pInvar': 0.5, 'nni': 1.0, 'learned': 1.0}
 
 
 class Mcmc(object):
@@ -400,13 +462,10 @@ class Mcmc(object):
             for ch, prop, lnl in zip(self.chains, props, lnLs):
                 ch.propTree.logLike = lnl
                 beta = 1.0 / (1.0 + self.chainTemps[ch.tempNum])
-                if self.samplePrior:
-                    pRet = beta * ch.logPriorRatio + ch.logProposalRatio
-                elif lnl <= -1.0e99 * 0.5:
-                    pRet = -1.0e300
-                else:
-                    pRet = beta * (lnl - ch.curTree.logLike) + beta * ch.logPriorRatio + ch.logProposalRatio
-                accept = pRet >= 0.0 or (pRet > -100.0 and self.rng.random() < math.exp(pRet))
+                pRet = heatedLogRatio(beta, lnl, ch.curTree.logLike, ch.logPriorRatio, ch.logProposalRatio,
+                                      self.samplePrior)
+                r = acceptProbability(pRet)
+                accept = r == 1.0 or self.rng.random() < r
                 prop.nProposals[ch.tempNum] += 1
                 prop.tnNSamples[ch.tempNum] += 1
                 if accept:
@@ -440,8 +499,10 @@ class Mcmc(object):
         self.swapMatrix[t1][t2] += 1
         b1 = 1.0 / (1.0 + self.chainTemps[t1])
         b2 = 1.0 / (1.0 + self.chainTemps[t2])
-        lnR = b1 * c2.curTree.logLike + b2 * c1.curTree.logLike - b1 * c1.curTree.logLike - b2 * c2.curTree.logLike
-        if lnR >= 0.0 or (lnR > -100.0 and self.rng.random() < math.exp(lnR)):
+        lnR = swapLogRatio(b1, b2, c1.curTree.logLike, c2.curTree.logLike)
+        # p4/mcmc.py:3324-3332: r = 0 below -100, 1 at or above 0
+        r = acceptProbability(lnR)
+        if self.rng.random() < r:
             self.swapMatrix[t2][t1] += 1
             c1.tempNum, c2.tempNum = t2, t1
 

@@ -10,8 +10,14 @@
             = the pending evaluations of 4 chains in one batched call
             (p4g_batchLogLike); full recompute and dirty-path variants.
 
+  het64     the 64-taxon, 50,000-site workload with one 4-state pruning
+            kernel variant forced (opsPerGroup as in p4gpu.autotune: -3
+            lean, -1 wave with scalar P, ...): full evaluations, the
+            pruning kernel's median device time; for counter runs.
+
     python -m p4gpu.measure prot128 [--steps 50]
     python -m p4gpu.measure multi48 [--steps 50]
+    python -m p4gpu.measure het64 --variant -3 --precision fp32 [--steps 4]
 """
 import argparse
 import json
@@ -91,6 +97,15 @@ def multi48(steps, warmup):
 
     out = dict(workload="multi48: 48 taxa, 24 partitions (16 DNA + 8 protein, %d patterns), 4 chains, FP32"
                % sum(p.nPatterns for p in data.parts))
+    # FP32 batch against FP64 single-tree evaluation of the same four states
+    v32 = full()
+    errs = []
+    for t, v in zip(trees, v32):
+        d = t.dupe()
+        d.precision = "fp64"
+        v64 = d.calcLogLike(verbose=0)
+        errs.append(abs(v - v64) / abs(v64))
+    out["lnL_rel_err_fp32_vs_fp64"] = max(errs)
     for name, fn in (("full", full), ("dirty_path", dirty)):
         for _ in range(warmup):
             fn()
@@ -102,13 +117,37 @@ def multi48(steps, warmup):
     return out
 
 
+def het64(steps, variant, precision):
+    pf.setPruneTuning(precision == "fp32", variant, 1)
+    tree, _ = synth.make_workload("het64", seed=1, precision=precision)
+    tree.calcLogLike(verbose=0)
+    c = tree.cTree
+    pf.setKernelTiming(True)
+    us = []
+    for _ in range(steps):
+        pf.p4_setPrams(c, -1)
+        lnl = pf.p4_treeLogLike(c, 0)
+        us.append(pf.lastKernelMicroseconds(c))
+    pf.setKernelTiming(False)
+    us.sort()
+    return dict(workload="het64", variant=variant, precision=precision, lnL=lnl,
+                kernel_us_median=us[len(us) // 2], kernel_us_min=us[0])
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["prot128", "multi48"])
+    ap.add_argument("what", choices=["prot128", "multi48", "het64"])
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--variant", type=int, default=-3)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     a = ap.parse_args(argv)
-    res = prot128(a.steps, a.warmup) if a.what == "prot128" else multi48(a.steps, a.warmup)
+    if a.what == "prot128":
+        res = prot128(a.steps, a.warmup)
+    elif a.what == "multi48":
+        res = multi48(a.steps, a.warmup)
+    else:
+        res = het64(a.steps, a.variant, a.precision)
     print(json.dumps(res))
 
 

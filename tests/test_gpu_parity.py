@@ -57,14 +57,25 @@ def test_golden_p_matrices(case):
         assert numpy.abs(got - fx["P"][n]).max() < 2e-14, n
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
-def test_partials_match_oracle(prec, tol):
-    """Every stored partial, unscaled, against the oracle's.  The error
+@pytest.mark.parametrize("case,prec,tol", [("het64_800", "fp64", 1e-10), ("het64_800", "fp32", 1e-4),
+                                           ("prot128_150", "fp64", 1e-10), ("prot128_150", "fp32", 1e-4),
+                                           ("prot128_150", "fp32-tile", 1e-4)])
+def test_partials_match_oracle(case, prec, tol):
+    """Every stored partial, unscaled, against the oracle's: 4-state (wave
+    or lean kernel) and 20-state (FP32 on MFMA with one wavefront per
+    (tile, category), or per tile; FP64 on the grouped kernel).  The error
     bound is relative to the largest entry of the same pattern (FP32 entries
     far below that largest one lose relative precision, harmlessly)."""
-    fx = load("het64_800")
-    t = tree_from_fixture(fx, prec)
-    t.calcLogLike(verbose=0)
+    fx = load(case)
+    tile = prec == "fp32-tile"
+    prec = "fp32" if tile else prec
+    if tile:
+        pf.setAaTuning(False)
+    try:
+        t = tree_from_fixture(fx, prec)
+        t.calcLogLike(verbose=0)
+    finally:
+        pf.setAaTuning(True)
     f = oracle_flat(fx)
     _, _, cl = oracle.oracle_cl(f)
     K, S, nP = int(fx["nCat"]), int(fx["dim"]), f.patterns.shape[1]
