@@ -220,6 +220,62 @@ int p4g_getBigPDeck(const p4g_node *n, int pNum, double *out);
  * unscaled: FP32 values are multiplied by 2^-exponent. */
 int p4g_getCondLikes(const p4g_node *n, int pNum, double *out);
 
+/* ---- branch-length derivatives (FP64 trees) ---------------------------- */
+/* For every non-root node n: the log likelihood computed across n's branch
+ * (equal to the tree's for every n), and its first and second derivatives
+ * with respect to n's branch length, summed over part pNum or all parts
+ * (pNum -1); the root's entries are 0.  Arrays [nNodes].  This is what the
+ * reference's Newton optimiser computes one node at a time:
+ * p4_calculateBigPDecks_1stD / _2ndD (Pf/p4_node.c:442-540, reached from
+ * pf.p4_newtSetup / p4_newtAndBrentPowellOpt, Pf/pfmodule.c:2259-2285),
+ * p4_setNodeCL2 with p4_initializeCL2ToRootComp / p4_setCL2Up /
+ * p4_setCL2Down (Pf/p4_treeNewt.c:597-620, Pf/p4_node.c:858-1018) and the
+ * per-site sums of p4_newtNode (Pf/p4_treeNewt.c:257-500); here for every
+ * branch in one launch per part.  Runs a full evaluation first. */
+int p4g_branchDerivatives(p4g_tree *t, int pNum, double *lnLEdge, double *d1, double *d2);
+/* The up-pass partials of non-root node n (node->cl2 of the reference,
+ * Pf/pftypes.h:134) as [nCat][dim][nPatterns], after p4g_branchDerivatives. */
+int p4g_getCondLikes2(const p4g_node *n, int pNum, double *out);
+/* dP/dt and d2P/dt2 of node n's branch (node->bigPDecks_1stD / _2ndD,
+ * Pf/pftypes.h:131-132) as [nCat][dim][dim], after p4g_branchDerivatives. */
+int p4g_getBigPDeck1stD(const p4g_node *n, int pNum, double *out);
+int p4g_getBigPDeck2ndD(const p4g_node *n, int pNum, double *out);
+
+/* ---- CPO / LPML (the doCpo step of Mcmc's sampling, p4/mcmc.py:3286-3297,
+ * and Mcmc.lpml, p4/mcmc.py:3730-3748).  p4g_cpoAccumulate evaluates tree t
+ * fully, as p4_treeLogLike(t, 1) (Pf/p4_tree.c:868-1021) does, keeps each
+ * pattern's likelihood in GPU memory and adds 1/like to every part's
+ * device-resident per-pattern sum; each part's sample count goes up by
+ * one.  lnL: the tree's log likelihood.  An error if a site likelihood is
+ * zero (the reference's 1./sL raises there).  The sums belong to the data
+ * parts, so every chain's tree adds to the same sums. */
+int p4g_cpoAccumulate(p4g_tree *t, double *lnL);
+/* cpo_sumsOfInverseSiteLikes for one part, by site: out [p4g_partNSites];
+ * nSamples: the number of accumulations. */
+int p4g_cpoGetSums(const p4g_part *p, double *out, int *nSamples);
+int p4g_cpoReset(p4g_part *p);
+/* Sites of a part: nChar, or the pattern count when the patterns were set
+ * without a site map. */
+int p4g_partNSites(const p4g_part *p);
+
+/* ---- ancestral states of the root: pf.p4_drawAncState (Pf/pfmodule.c:
+ * 2353-2367 -> p4_drawAncStateP, Pf/p4_treeSim.c:591-857) for the sites
+ * [first, first + n) of part pNum in one launch.  Each site draws a
+ * (gamma category, state) or an invariant state from the table the
+ * reference builds from root->cl: comp[s] * cl[cat][s] * (1 - pInvar)/nCat,
+ * category-major, then comp[s] * pInvar for the site's invariant states;
+ * the pick is the first entry whose running sum exceeds u * total.
+ * u: n uniforms in [0, 1) (the reference's random() / (RAND_MAX + 1.0)),
+ * or NULL: u from Philox4x64-10 keyed by (seed, site).
+ * draw: [n][4] = chStNum, catNum, isInvar, invarChNum (Pf/p4_treeSim.c:
+ * 821-840).  The partials are brought up to date first. */
+int p4g_drawAncStates(p4g_tree *t, int pNum, int first, int n, const double *u, unsigned long long seed,
+                      int *draw);
+/* The counter-based generator of the GPU draws (no counterpart in the
+ * reference, whose draws use random() and GSL): Philox4x64-10 of counter[4]
+ * under key[2], on the host, from the same source as the device code. */
+void p4g_philox4x64(const unsigned long long *counter, const unsigned long long *key, unsigned long long *out);
+
 /* ---- batched evaluation (no single counterpart: replaces the sequential
  * per-chain, per-part p4_partLogLike calls of Mcmc.run, p4/mcmc.py:2861-2950) */
 /* A batch evaluates the pending partials and site sums of several trees

@@ -108,6 +108,8 @@ def ref():
         _ref.ref_bigq_pmat.restype = I
         _ref.ref_part_loglike.argtypes = [P, P, P, P, P, P, P, P]
         _ref.ref_part_loglike.restype = D
+        _ref.ref_part_derivs.argtypes = [P] * 11
+        _ref.ref_part_derivs.restype = I
     return _ref
 
 
@@ -311,6 +313,27 @@ def oracle_cl(f):
     return v, Pm, cl
 
 
+def ref_part_derivs(f, want_cl2=()):
+    """Branch-length derivatives of the partition's lnL through the compiled
+    reference subset (its P', P'' and up-pass conditional likelihoods, and
+    the restated per-site sums of Pf/p4_treeNewt.c:257-500).  Returns
+    (lnLEdge [N], d1 [N], d2 [N], P1 [N][K][S][S], P2, cl2 {node: [K][S][nP]})."""
+    r = ref()
+    if r is None:
+        raise RuntimeError("reference subset not built")
+    K, dim, N, nP = f.nCat, f.dim, len(f.parent), f.patterns.shape[1]
+    want = numpy.zeros(N, numpy.int32)
+    for n in want_cl2:
+        want[n] = 1
+    cl2 = numpy.zeros((N, K, dim, nP)) if len(want_cl2) else None
+    P1 = numpy.zeros((N, K, dim, dim))
+    P2 = numpy.zeros((N, K, dim, dim))
+    lnl, d1, d2 = numpy.zeros(N), numpy.zeros(N), numpy.zeros(N)
+    r.ref_part_derivs(ctypes.byref(f.tr), ctypes.byref(f.mp), ctypes.byref(f.dp), _p(f.alphas), _p(want),
+                      _p(cl2), _p(P1), _p(P2), _p(lnl), _p(d1), _p(d2))
+    return lnl, d1, d2, P1, P2, ({n: cl2[n] for n in want_cl2} if cl2 is not None else {})
+
+
 def ref_part_loglike(f, want_cl=False):
     """Same partition through the compiled reference subset.  Returns
     (lnL, patternLikes, P, cl or None, rates)."""
@@ -325,3 +348,86 @@ def ref_part_loglike(f, want_cl=False):
     v = r.ref_part_loglike(ctypes.byref(f.tr), ctypes.byref(f.mp), ctypes.byref(f.dp),
                            _p(f.alphas), _p(Pm), _p(cl), _p(rates), _p(pl))
     return v, pl, Pm, cl, rates
+
+
+# ------------------------------------------------- root-side consumers
+def anc_table(f, cl_root):
+    """The picker table of Pf/p4_treeSim.c:665-692 for every pattern:
+    entries (cat, state), category-major, each comp[s] * cl[cat][s][p] *
+    (1 - pInvar)/nCat with comp the root's composition, then one entry per
+    state, comp[s] * pInvar where the pattern is invariant in that state
+    (0 elsewhere; the reference skips those, which leaves the running sums
+    unchanged).  cl_root: [K][S][nPat].  Returns [nPat][K*S + S] weights."""
+    K, S = f.nCat, f.dim
+    comp = f.comps[f.compNums[f.root]]
+    fac = (1.0 - f.pInvar) / K
+    nP = cl_root.shape[2]
+    W = numpy.zeros((nP, K * S + S))
+    for k in range(K):
+        for s in range(S):
+            W[:, k * S + s] = (comp[s] * cl_root[k, s, :]) * fac
+    if f.pInvar != 0.0:
+        for s in range(S):
+            W[:, K * S + s] = numpy.where(f.invarArray[s] != 0, comp[s] * f.pInvar, 0.0)
+    return W
+
+
+def anc_draw(f, cl_root, site_pat, u):
+    """Pf/p4_treeSim.c:591-845 for each site: theRanDouble = u * sLike, the
+    first table entry whose running sum exceeds it.  Returns [nSites][4]
+    (chStNum, catNum, isInvar, invarChNum), and the running sums
+    [nSites][K*S + S] for the checks on ties."""
+    K, S = f.nCat, f.dim
+    W = anc_table(f, cl_root)[numpy.asarray(site_pat)]
+    cum = numpy.cumsum(W, axis=1)      # left to right: the reference's running sum
+    target = numpy.asarray(u) * cum[:, -1]
+    hit = target[:, None] < cum
+    idx = numpy.where(hit.any(axis=1), hit.argmax(axis=1), -1)
+    out = numpy.full((len(idx), 4), -1, numpy.int32)
+    out[:, 2] = 0
+    var = (idx >= 0) & (idx < K * S)
+    out[var, 0] = idx[var] % S
+    out[var, 1] = idx[var] // S
+    inv = idx >= K * S
+    out[inv, 2] = 1
+    out[inv, 3] = idx[inv] - K * S
+    return out, cum
+
+
+def philox4x64(counter, key):
+    """Philox4x64-10 of one counter (4 words) and key (2 words), through
+    numpy's Philox bit generator (which increments its counter before each
+    block of four outputs)."""
+    c = numpy.array(counter, dtype=numpy.uint64) - numpy.array([1, 0, 0, 0], dtype=numpy.uint64)
+    if int(counter[0]) == 0:   # borrow through the words, as the generator carries
+        c = numpy.array(counter, dtype=numpy.uint64)
+        for i in range(4):
+            if c[i] != 0:
+                c[i] -= numpy.uint64(1)
+                break
+            c[i] = numpy.uint64(0xFFFFFFFFFFFFFFFF)
+    bg = numpy.random.Philox(key=numpy.array(key, dtype=numpy.uint64), counter=c)
+    return bg.random_raw(4)
+
+
+def anc_uniforms(seed, sites):
+    """The uniforms the GPU draws take when none are given: the first word of
+    Philox4x64-10 at counter (site, 'anc', 0, 0), key (seed, 'root'), top
+    53 bits."""
+    return numpy.array([float(int(philox4x64([s, 0x616e63, 0, 0], [seed, 0x726f6f74])[0]) >> 11) * 2.0 ** -53
+                        for s in sites])
+
+
+def cpo_sums(site_like_samples):
+    """p4/mcmc.py:3286-3297: per site, the sum over samples of 1/siteLike."""
+    acc = numpy.zeros(len(site_like_samples[0]))
+    for sl in site_like_samples:
+        for i, v in enumerate(sl):
+            acc[i] += 1.0 / v
+    return acc
+
+
+def lpml(sums, nSamples):
+    """p4/mcmc.py:3742-3746."""
+    a = numpy.array(sums) / nSamples
+    return float((-numpy.log(a)).sum())

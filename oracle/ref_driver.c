@@ -96,9 +96,90 @@ typedef struct {
  *   patternLikes: [nPatterns]          (may be NULL)
  * Returns lnL (or -1e99 as the reference does).
  */
+typedef struct {
+    p4_tree *tree;
+    p4_model *model;
+    part *aPart;
+    int *bQET;
+} ref_built;
+
+/* Build the reference structs for one part, run p4_setPramsPart's
+ * orchestration and the postorder conditional likelihoods. */
+static ref_built ref_build(const or_tree *tr, const or_modelpart *mp, const or_part *dp, const double *alphas,
+                           double *outRates);
+
 double ref_part_loglike(const or_tree *tr, const or_modelpart *mp, const or_part *dp,
                         const double *alphas, double *outP, double *outCl, double *outRates,
                         double *patternLikes)
+{
+    int dim = mp->dim, K = mp->nCat, nP = dp->nPatterns, n, i, j, k, s;
+    double lnL = 0.0, like, fTOMP;
+    const double *pi;
+    p4_node *root;
+    ref_built b = ref_build(tr, mp, dp, alphas, outRates);
+    p4_tree *aTree = b.tree;
+    p4_modelPart *pm = b.model->parts[0];
+    part *aPart = b.aPart;
+    root = aTree->root;
+    if (outP)
+        for (n = 0; n < tr->nNodes; n++) {
+            if (aTree->nodes[n] == root) continue;
+            for (k = 0; k < K; k++)
+                for (i = 0; i < dim; i++)
+                    for (j = 0; j < dim; j++)
+                        outP[(((size_t)n * K + k) * dim + i) * dim + j] = aTree->nodes[n]->bigPDecks[0][k][i][j];
+        }
+    if (outCl)
+        for (n = 0; n < tr->nNodes; n++) {
+            if (aTree->nodes[n]->cl == NULL) continue;
+            for (k = 0; k < K; k++)
+                for (i = 0; i < dim; i++)
+                    for (j = 0; j < nP; j++)
+                        outCl[(((size_t)n * K + k) * dim + i) * nP + j] = aTree->nodes[n]->cl[0][k][i][j];
+        }
+
+    /* root site sum (Pf/p4_tree.c:1029-1197 and 1199-1378) */
+    pi = pm->comps[root->compNums[0]]->val;
+    fTOMP = (1.0 - pm->pInvar->val[0]) / (double)K;
+    for (j = 0; j < nP; j++) {
+        like = 0.0;
+        for (k = 0; k < K; k++) {
+            if (root->isLeaf) {
+                int code = aPart->patterns[root->seqNum][j];
+                int isN = 0;
+                if (code >= EQUATES_BASE && code < EQUATES_BASE + aPart->nEquates) {
+                    isN = 1;
+                    for (s = 0; s < dim; s++)
+                        if (!aPart->equates[code - EQUATES_BASE][s]) isN = 0;
+                }
+                if (code >= 0)
+                    like += pi[code] * root->cl[0][k][code][j];
+                else if (code == GAP_CODE || code == QMARK_CODE || isN)
+                    for (s = 0; s < dim; s++) like += pi[s] * root->cl[0][k][s][j];
+                else
+                    for (s = 0; s < dim; s++)
+                        if (aPart->equates[code - EQUATES_BASE][s]) like += pi[s] * root->cl[0][k][s][j];
+            } else {
+                for (s = 0; s < dim; s++) like += pi[s] * root->cl[0][k][s][j];
+            }
+        }
+        if (pm->pInvar->val[0]) {
+            like *= fTOMP;
+            if (aPart->globalInvarSitesVec[j] > 0)
+                for (s = 0; s < dim; s++)
+                    if (aPart->globalInvarSitesArray[s][j]) like += pi[s] * pm->pInvar->val[0];
+        } else if (K > 1) {
+            like = like / (double)K;
+        }
+        if (like <= 0.0) return -1.0e99;
+        lnL = lnL + (aPart->patternCounts[j] * log(like));
+        if (patternLikes) patternLikes[j] = like;
+    }
+    return lnL;
+}
+
+static ref_built ref_build(const or_tree *tr, const or_modelpart *mp, const or_part *dp, const double *alphas,
+                           double *outRates)
 {
     static int iOne = 1;
     static double PINVAR_MIN = 0.0, PINVAR_MAX = 0.99, KAPPA_MIN = 0.000001, KAPPA_MAX = 100.0,
@@ -228,59 +309,182 @@ double ref_part_loglike(const or_tree *tr, const or_modelpart *mp, const or_part
         nd = aTree->nodes[ni];
         if (!nd->isLeaf || (nd == root && nd->isLeaf)) p4_setConditionalLikelihoodsOfInternalNode(nd);
     }
-    if (outP)
-        for (n = 0; n < tr->nNodes; n++) {
-            if (aTree->nodes[n] == root) continue;
-            for (k = 0; k < K; k++)
-                for (i = 0; i < dim; i++)
-                    for (j = 0; j < dim; j++)
-                        outP[(((size_t)n * K + k) * dim + i) * dim + j] = aTree->nodes[n]->bigPDecks[0][k][i][j];
-        }
-    if (outCl)
-        for (n = 0; n < tr->nNodes; n++) {
-            if (aTree->nodes[n]->cl == NULL) continue;
-            for (k = 0; k < K; k++)
-                for (i = 0; i < dim; i++)
-                    for (j = 0; j < nP; j++)
-                        outCl[(((size_t)n * K + k) * dim + i) * nP + j] = aTree->nodes[n]->cl[0][k][i][j];
-        }
+    {
+        ref_built b;
+        b.tree = aTree;
+        b.model = model;
+        b.aPart = aPart;
+        b.bQET = bQET;
+        (void)lnL;
+        (void)like;
+        (void)fTOMP;
+        (void)pi;
+        (void)partLikes;
+        return b;
+    }
+}
 
-    /* root site sum (Pf/p4_tree.c:1029-1197 and 1199-1378) */
-    pi = pm->comps[root->compNums[0]]->val;
-    fTOMP = (1.0 - pm->pInvar->val[0]) / (double)K;
-    for (j = 0; j < nP; j++) {
-        like = 0.0;
+
+/*
+ * Branch-length derivatives through the reference's own functions:
+ *   p4_calculateBigPDecks_1stD / _2ndD      Pf/p4_node.c:442-540
+ *   p4_initializeCL2ToRootComp, p4_setCL2Up,
+ *   p4_setCL2Down                           Pf/p4_node.c:858-1018
+ * p4_treeNewt.c cannot be compiled here (it includes p4_tree.h, which
+ * includes <gsl/gsl_rng.h>), so the driver restates its p4_setNodeCL2
+ * dispatch (Pf/p4_treeNewt.c:597-620) and the per-site derivative sums of
+ * p4_newtNode (Pf/p4_treeNewt.c:257-500) around them.
+ *   outCl2 : [nNodes][nCat][dim][nPatterns] for the nodes in wantCl2 (may be NULL)
+ *   outP1, outP2 : [nNodes][nCat][dim][dim] (may be NULL)
+ *   outLnL, outD1, outD2 : [nNodes], per non-root node: the part's log
+ *   likelihood computed across that node's branch, and its first and
+ *   second derivatives with respect to the node's branch length.
+ */
+int ref_part_derivs(const or_tree *tr, const or_modelpart *mp, const or_part *dp, const double *alphas,
+                    const int *wantCl2, double *outCl2, double *outP1, double *outP2, double *outLnL,
+                    double *outD1, double *outD2)
+{
+    int dim = mp->dim, K = mp->nCat, nP = dp->nPatterns, n, i, j, k, from, to, q;
+    ref_built b = ref_build(tr, mp, dp, alphas, NULL);
+    p4_tree *aTree = b.tree;
+    p4_modelPart *pm = b.model->parts[0];
+    part *aPart = b.aPart;
+    p4_node *root = aTree->root;
+    int *pre = (int *)malloc(sizeof(int) * tr->nNodes);
+    int nPre = 0;
+    for (n = 0; n < tr->nNodes; n++) {
+        p4_node *nd = aTree->nodes[n];
+        nd->cl2 = (double ****)malloc(sizeof(double ***));
+        nd->cl2[0] = (double ***)malloc(K * sizeof(double **));
+        nd->bigPDecks_1stD = (double ****)malloc(sizeof(double ***));
+        nd->bigPDecks_1stD[0] = (double ***)malloc(K * sizeof(double **));
+        nd->bigPDecks_2ndD = (double ****)malloc(sizeof(double ***));
+        nd->bigPDecks_2ndD[0] = (double ***)malloc(K * sizeof(double **));
         for (k = 0; k < K; k++) {
-            if (root->isLeaf) {
-                int code = aPart->patterns[root->seqNum][j];
-                int isN = 0;
-                if (code >= EQUATES_BASE && code < EQUATES_BASE + aPart->nEquates) {
-                    isN = 1;
-                    for (s = 0; s < dim; s++)
-                        if (!aPart->equates[code - EQUATES_BASE][s]) isN = 0;
+            nd->cl2[0][k] = pdmatrix(dim, nP);
+            nd->bigPDecks_1stD[0][k] = psdmatrix(dim);
+            nd->bigPDecks_2ndD[0][k] = psdmatrix(dim);
+        }
+        if (nd != root) {
+            p4_calculateBigPDecks_1stD(nd);
+            p4_calculateBigPDecks_2ndD(nd);
+        }
+    }
+    /* preorder: the reverse of the postorder */
+    for (j = tr->nNodes - 1; j >= 0; j--)
+        if (tr->postOrder[j] != NO_ORDER) pre[nPre++] = tr->postOrder[j];
+    for (q = 0; q < nPre; q++) {
+        p4_node *nd = aTree->nodes[pre[q]], *p;
+        if (nd == root) continue;
+        /* p4_setNodeCL2 */
+        if (nd->parent == root)
+            p4_initializeCL2ToRootComp(nd);
+        else
+            p4_setCL2Up(nd);
+        for (p = nd->parent->leftChild; p; p = p->sibling)
+            if (p != nd) p4_setCL2Down(nd, p);
+    }
+    for (n = 0; n < tr->nNodes; n++) {
+        p4_node *nd = aTree->nodes[n];
+        double lnL = 0.0, firstD = 0.0, secondD = 0.0;
+        double fTOMP = (1.0 - pm->pInvar->val[0]) / (double)K;
+        if (nd == root) {
+            outLnL[n] = outD1[n] = outD2[n] = 0.0;
+            continue;
+        }
+        for (q = 0; q < nP; q++) {
+            double likeS = 0.0, firstS = 0.0, secondS = 0.0;
+            for (k = 0; k < K; k++) {
+                double like = 0.0, first = 0.0, second = 0.0, t2;
+                double **P = nd->bigPDecks[0][k], **P1 = nd->bigPDecks_1stD[0][k], **P2 = nd->bigPDecks_2ndD[0][k];
+                double **c2 = nd->cl2[0][k];
+                if (nd->isLeaf) {
+                    int code = aPart->patterns[nd->seqNum][q];
+                    if (code >= 0) {
+                        for (from = 0; from < dim; from++) {
+                            t2 = c2[from][q];
+                            like += t2 * P[from][code];
+                            first += t2 * P1[from][code];
+                            second += t2 * P2[from][code];
+                        }
+                    } else if (code == GAP_CODE || code == QMARK_CODE) {
+                        for (from = 0; from < dim; from++) {
+                            t2 = c2[from][q];
+                            like += t2;
+                            for (to = 0; to < dim; to++) {
+                                first += t2 * P1[from][to];
+                                second += t2 * P2[from][to];
+                            }
+                        }
+                    } else {
+                        int e = code - EQUATES_BASE, isN = 1;
+                        for (from = 0; from < dim; from++)
+                            if (!aPart->equates[e][from]) isN = 0;
+                        for (from = 0; from < dim; from++) {
+                            t2 = c2[from][q];
+                            for (to = 0; to < dim; to++)
+                                if (isN || aPart->equates[e][to]) {
+                                    like += t2 * P[from][to];
+                                    first += t2 * P1[from][to];
+                                    second += t2 * P2[from][to];
+                                }
+                        }
+                    }
+                } else {
+                    for (from = 0; from < dim; from++)
+                        for (to = 0; to < dim; to++) {
+                            t2 = c2[from][q] * nd->cl[0][k][to][q];
+                            like += t2 * P[from][to];
+                            first += t2 * P1[from][to];
+                            second += t2 * P2[from][to];
+                        }
                 }
-                if (code >= 0)
-                    like += pi[code] * root->cl[0][k][code][j];
-                else if (code == GAP_CODE || code == QMARK_CODE || isN)
-                    for (s = 0; s < dim; s++) like += pi[s] * root->cl[0][k][s][j];
-                else
-                    for (s = 0; s < dim; s++)
-                        if (aPart->equates[code - EQUATES_BASE][s]) like += pi[s] * root->cl[0][k][s][j];
+                likeS += like;
+                firstS += first;
+                secondS += second;
+            }
+            if (pm->pInvar->val[0]) {
+                likeS *= fTOMP;
+                firstS *= fTOMP;
+                secondS *= fTOMP;
+                if (aPart->globalInvarSitesVec[q] > 0)
+                    for (from = 0; from < dim; from++)
+                        if (aPart->globalInvarSitesArray[from][q])
+                            likeS += pm->comps[root->compNums[0]]->val[from] * pm->pInvar->val[0];
+            } else if (K > 1) {
+                likeS /= (double)K;
+                firstS /= (double)K;
+                secondS /= (double)K;
+            }
+            if (likeS < 1.0e-300) {
+                lnL += aPart->patternCounts[q] * -100000;
+                firstD += aPart->patternCounts[q] * 1000000;
+                secondD += aPart->patternCounts[q] * 10000000;
             } else {
-                for (s = 0; s < dim; s++) like += pi[s] * root->cl[0][k][s][j];
+                lnL += aPart->patternCounts[q] * log(likeS);
+                firstD += aPart->patternCounts[q] * (firstS / likeS);
+                secondD += aPart->patternCounts[q] * ((secondS * likeS - firstS * firstS) / (likeS * likeS));
             }
         }
-        if (pm->pInvar->val[0]) {
-            like *= fTOMP;
-            if (aPart->globalInvarSitesVec[j] > 0)
-                for (s = 0; s < dim; s++)
-                    if (aPart->globalInvarSitesArray[s][j]) like += pi[s] * pm->pInvar->val[0];
-        } else if (K > 1) {
-            like = like / (double)K;
-        }
-        if (like <= 0.0) return -1.0e99;
-        lnL = lnL + (aPart->patternCounts[j] * log(like));
-        if (patternLikes) patternLikes[j] = like;
+        outLnL[n] = lnL;
+        outD1[n] = firstD;
+        outD2[n] = secondD;
     }
-    return lnL;
+    for (n = 0; n < tr->nNodes; n++) {
+        p4_node *nd = aTree->nodes[n];
+        for (k = 0; k < K; k++) {
+            if (outP1 && nd != root)
+                for (i = 0; i < dim; i++)
+                    for (j = 0; j < dim; j++) {
+                        outP1[(((size_t)n * K + k) * dim + i) * dim + j] = nd->bigPDecks_1stD[0][k][i][j];
+                        outP2[(((size_t)n * K + k) * dim + i) * dim + j] = nd->bigPDecks_2ndD[0][k][i][j];
+                    }
+            if (outCl2 && wantCl2 && wantCl2[n] && nd != root)
+                for (i = 0; i < dim; i++)
+                    for (j = 0; j < nP; j++)
+                        outCl2[(((size_t)n * K + k) * dim + i) * nP + j] = nd->cl2[0][k][i][j];
+        }
+    }
+    free(pre);
+    return 0;
 }

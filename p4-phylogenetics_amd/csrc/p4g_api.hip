@@ -140,6 +140,8 @@ Part::~Part()
     devFree(dev.counts);
     devFree(dev.invMask);
     devFree(dev.ambMask);
+    devFree(dev.sitePat);
+    devFree(cpoSum);
 }
 
 ModelPart::~ModelPart()
@@ -176,6 +178,21 @@ static void freeTreePart(TreePart &t)
     if (t.ev0) (void)hipEventDestroy(t.ev0);
     if (t.ev1) (void)hipEventDestroy(t.ev1);
     t.ev0 = t.ev1 = nullptr;
+    devFree(t.cl2);
+    devFree(t.P1T);
+    devFree(t.P2T);
+    devFree(t.dQ);
+    devFree(t.dRate);
+    devFree(t.dPartial);
+    devFree(t.dOut);
+    devFree(t.dItems);
+    if (t.dProg) (void)hipFree(t.dProg);
+    t.dProg = nullptr;
+    t.dPartialCap = t.dProgCap = 0;
+    if (t.ancBuf) (void)hipFree(t.ancBuf);
+    t.ancBuf = nullptr;
+    t.ancCap = 0;
+    t.derivValid = false;
     t.alloc = false;
 }
 
@@ -252,6 +269,15 @@ static bool uploadPart(Part *p)
     if (!checkCodes(p)) return false;
     if (!p->invarSet) computeInvar(p);
     DevPart &d = p->dev;
+    if (d.tips || d.counts || d.invMask || d.ambMask || d.sitePat) {
+        // re-upload after the part changed: the sizes may differ
+        if (!hipOk(hipStreamSynchronize(stream()), "sync before re-upload")) return false;
+        devFree(d.tips);
+        devFree(d.counts);
+        devFree(d.invMask);
+        devFree(d.ambMask);
+        devFree(d.sitePat);
+    }
     const int nP = p->nPatterns;
     const int nPatPad = (nP + 255) / 256 * 256;
     d.nPatPad = nPatPad;
@@ -293,13 +319,28 @@ static bool uploadPart(Part *p)
     for (int j = 0; j < p->dim; j++)
         for (int k = 0; k < nP; k++)
             if (p->invarArray[(size_t)j * nP + k]) inv[(size_t)(j / 32) * nPatPad + k] |= (1u << (j % 32));
+    // the pattern of each site (sites are patterns when no site map is set)
+    std::vector<int> sitePat;
+    if (p->nChar > 0 && (int)p->seqPosPatternIndex.size() == p->nChar)
+        sitePat = p->seqPosPatternIndex;
+    else
+        for (int k = 0; k < nP; k++) sitePat.push_back(k);
+    d.nSites = (int)sitePat.size();
+    for (int v : sitePat)
+        if (v < 0 || v >= nP) {
+            setError("a site maps to an out-of-range pattern");
+            return false;
+        }
     if (!devAlloc(&d.tips, tips.size(), "hipMalloc tips") ||
         !devAlloc(&d.counts, counts.size(), "hipMalloc counts") ||
         !devAlloc(&d.invMask, inv.size(), "hipMalloc invMask") ||
-        !devAlloc(&d.ambMask, std::max<size_t>(d.hostAmb.size(), 1), "hipMalloc ambMask"))
+        !devAlloc(&d.ambMask, std::max<size_t>(d.hostAmb.size(), 1), "hipMalloc ambMask") ||
+        !devAlloc(&d.sitePat, sitePat.size(), "hipMalloc site patterns"))
         return false;
     hipStream_t s = stream();
-    if (!hipOk(hipMemcpyAsync(d.tips, tips.data(), tips.size(), hipMemcpyHostToDevice, s), "copy tips") ||
+    if (!hipOk(hipMemcpyAsync(d.sitePat, sitePat.data(), sitePat.size() * 4, hipMemcpyHostToDevice, s),
+               "copy site patterns") ||
+        !hipOk(hipMemcpyAsync(d.tips, tips.data(), tips.size(), hipMemcpyHostToDevice, s), "copy tips") ||
         !hipOk(hipMemcpyAsync(d.counts, counts.data(), counts.size() * 8, hipMemcpyHostToDevice, s),
                "copy counts") ||
         !hipOk(hipMemcpyAsync(d.invMask, inv.data(), inv.size() * 4, hipMemcpyHostToDevice, s),
@@ -330,6 +371,12 @@ static bool allocTreePart(Tree *t, int pNum)
     Part *dp = t->data->parts[pNum];
     if (!uploadPart(dp)) return false;
     const bool wave = pruneUsesWave(mp->dim, mp->nCat, t->precision);
+    if (tp.alloc && (tp.nPat != dp->nPatterns || tp.nPatPad != dp->dev.nPatPad)) {
+        setError("part " + std::to_string(pNum) +
+                 ": the data part's patterns changed after the tree's device state was made; free the tree "
+                 "and build it again");
+        return false;
+    }
     if (tp.alloc && tp.precision == t->precision && tp.wave == wave) return true;
     if (tp.alloc) {
         // precision or kernel change: the partial buffers are re-laid out
@@ -1349,14 +1396,17 @@ static void finishPart(Tree *t, int pNum, bool siteLikes, double lnL, bool bad)
 // kernel's reduction scratch is taken from there, the counter is cleared by
 // a fill kernel first, and the wave kernel reads P^T by node (no staged
 // program-order P stream).
-static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL, unsigned char *scr = nullptr)
+// With `cpoSum` (device, one double per pattern) 1/like of every pattern is
+// added to it after the pruning kernel, on the same stream.
+static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL, unsigned char *scr = nullptr,
+                     double *cpoSum = nullptr)
 {
     TreePart &tp = t->tp[pNum];
     Part *dp = t->data->parts[pNum];
     int nOps = 0;
     if (!stagePart(t, pNum, full, &nOps)) return false;
     PruneArgs a{};
-    fillPruneArgs(t, pNum, nOps, a, siteLikes);
+    fillPruneArgs(t, pNum, nOps, a, siteLikes || cpoSum);
     hipStream_t s = stream();
     const int nBlocks = tp.wave ? tp.nPatPad / pruneWaveTilePatterns(t->precision)
                                 : tp.nPatPad / pruneBlockPatternsFor(a.dim, a.K, t->precision, nOps);
@@ -1394,6 +1444,10 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL, 
         return false;
     }
     if (g_timing) (void)hipEventRecord(tp.ev1, s);
+    if (cpoSum && !hipOk(launchCpoAccum(tp.patLikes, cpoSum, tp.nPat, s), "launch CPO sums")) {
+        (void)hipStreamSynchronize(s);
+        return false;
+    }
     std::vector<double> pl;
     if (siteLikes) {
         pl.resize(tp.nPat);
@@ -1420,6 +1474,164 @@ static bool evalPart(Tree *t, int pNum, bool full, bool siteLikes, double *lnL, 
         }
     }
     finishPart(t, pNum, siteLikes, *lnL, bad);
+    return true;
+}
+
+// Q of Pf/util.c:44-115 (setBigQFromRMatrixDotCharFreq, normalizeBigQ), in
+// the reference's operation order.
+static void hostBigQ(int S, const double *pi, const double *R, double *Q)
+{
+    for (int c = 0; c < S; c++)
+        for (int r = 0; r < S; r++) Q[r * S + c] = R[r * S + c] * pi[c];
+    for (int r = 0; r < S; r++) {
+        double sum = 0.0;
+        for (int c = 0; c < S; c++)
+            if (r != c) sum = sum + Q[r * S + c];
+        Q[r * S + r] = -sum;
+    }
+    double sumODE = 0.0;
+    for (int r = 0; r < S; r++)
+        for (int c = 0; c < S; c++)
+            if (r != c) sumODE = sumODE + pi[r] * Q[r * S + c];
+    sumODE = 1.0 / sumODE;
+    for (int i = 0; i < S * S; i++) Q[i] = Q[i] * sumODE;
+}
+
+// Per non-root node of part pNum: lnL across its branch and its first and
+// second derivatives with respect to the branch length (out [nNodes][3]).
+// The partials and P matrices must be current (a full evaluation just ran).
+static bool derivPart(Tree *t, int pNum, std::vector<double> &out)
+{
+    TreePart &tp = t->tp[pNum];
+    ModelPart *mp = t->model->parts[pNum];
+    Part *dp = t->data->parts[pNum];
+    const int S = tp.dim, K = tp.K, N = t->nNodes;
+    hipStream_t s = stream();
+    if (K > DERIV_MAX_K) {
+        setError("branch derivatives: at most 16 gamma categories");
+        return false;
+    }
+    const size_t SS = (size_t)S * S;
+    if (!devAlloc(&tp.cl2, (size_t)N * K * tp.nPatPad * S, "hipMalloc cl2") ||
+        !devAlloc(&tp.P1T, (size_t)N * K * SS, "hipMalloc P'") || !devAlloc(&tp.P2T, (size_t)N * K * SS, "hipMalloc P''") ||
+        !devAlloc(&tp.dQ, (size_t)std::max(tp.nEig, 1) * SS, "hipMalloc Q") ||
+        !devAlloc(&tp.dRate, (size_t)N * K, "hipMalloc rate factors") ||
+        !devAlloc(&tp.dItems, (size_t)2 * N, "hipMalloc derivative items") ||
+        !devAlloc(&tp.dOut, (size_t)3 * N, "hipMalloc derivative output"))
+        return false;
+    // Q of every (comp, rMatrix) slot
+    std::vector<double> Q((size_t)std::max(tp.nEig, 1) * SS, 0.0);
+    for (int c = 0; c < mp->nComps; c++)
+        for (int r = 0; r < mp->nRMatrices; r++)
+            hostBigQ(S, mp->comps[c].val, mp->rMatrices[r].bigR.data(), Q.data() + (size_t)(c * mp->nRMatrices + r) * SS);
+    // every non-root node: its slot and d(effective length)/d(branch length)
+    // (Pf/p4_node.c:442-540 without the branch length)
+    std::vector<int> items(2 * N);
+    std::vector<double> rate((size_t)N * K);
+    int nI = 0;
+    for (int n = 0; n < N; n++) {
+        if (n == t->root) continue;
+        const Node *nd = t->nodes[n];
+        items[nI] = n;
+        items[N + nI] = nd->compNums[pNum] * mp->nRMatrices + nd->rMatrixNums[pNum];
+        for (int k = 0; k < K; k++) {
+            const double g = mp->nGdasrvs ? mp->gdasrvs[nd->gdasrvNums[pNum]]->rates[k] : 1.0;
+            const double r = g * mp->relRate;
+            rate[(size_t)nI * K + k] = mp->pInvar == 0.0 ? r : r / (1.0 - mp->pInvar);
+        }
+        nI++;
+    }
+    // the up-pass program: non-root nodes in preorder with their siblings
+    std::vector<DerivOp> ops;
+    std::vector<DerivSib> sibs;
+    for (int j = N - 1; j >= 0; j--) {
+        const int n = t->postOrder[j];
+        if (n == NO_ORDER || n == t->root) continue;
+        if (n < 0 || n >= N) {
+            setError("postOrder holds an out-of-range node number");
+            return false;
+        }
+        const Node *nd = t->nodes[n];
+        DerivOp op{};
+        op.node = n;
+        op.flags = (nd->isLeaf ? DOP_LEAF : 0) | (nd->parent == t->root ? DOP_PARENT_ROOT : 0);
+        op.idx = nd->isLeaf ? nd->seqNum : t->slotOf[n];
+        op.parent = nd->parent;
+        op.sibStart = (int)sibs.size();
+        for (int c = t->nodes[nd->parent]->leftChild; c >= 0; c = t->nodes[c]->sibling) {
+            if (c == n) continue;
+            const Node *cn = t->nodes[c];
+            sibs.push_back(DerivSib{cn->isLeaf ? 0 : 1, cn->isLeaf ? cn->seqNum : t->slotOf[c], c, 0});
+        }
+        op.nSib = (int)sibs.size() - op.sibStart;
+        ops.push_back(op);
+    }
+    const int nOps = (int)ops.size();
+    const int nBlocks = tp.nPatPad / 64;
+    const size_t progBytes = ops.size() * sizeof(DerivOp) + std::max<size_t>(sibs.size(), 1) * sizeof(DerivSib);
+    if (progBytes > tp.dProgCap) {
+        if (tp.dProg) (void)hipFree(tp.dProg);
+        tp.dProg = nullptr;
+        if (!hipOk(hipMalloc(&tp.dProg, progBytes), "hipMalloc derivative program")) return false;
+        tp.dProgCap = progBytes;
+    }
+    const size_t partN = (size_t)std::max(nOps, 1) * nBlocks * 3;
+    if (partN > tp.dPartialCap) {
+        devFree(tp.dPartial);
+        if (!devAlloc(&tp.dPartial, partN, "hipMalloc derivative sums")) return false;
+        tp.dPartialCap = partN;
+    }
+    DerivOp *dOps = reinterpret_cast<DerivOp *>(tp.dProg);
+    DerivSib *dSibs = reinterpret_cast<DerivSib *>(dOps + ops.size());
+    if (!hipOk(hipMemcpyAsync(tp.dQ, Q.data(), Q.size() * 8, hipMemcpyHostToDevice, s), "stage Q") ||
+        !hipOk(hipMemcpyAsync(tp.dRate, rate.data(), rate.size() * 8, hipMemcpyHostToDevice, s), "stage rates") ||
+        !hipOk(hipMemcpyAsync(tp.dItems, items.data(), items.size() * 4, hipMemcpyHostToDevice, s), "stage items") ||
+        !hipOk(hipMemcpyAsync(dOps, ops.data(), ops.size() * sizeof(DerivOp), hipMemcpyHostToDevice, s), "stage program") ||
+        (!sibs.empty() &&
+         !hipOk(hipMemcpyAsync(dSibs, sibs.data(), sibs.size() * sizeof(DerivSib), hipMemcpyHostToDevice, s),
+                "stage siblings"))) {
+        (void)hipStreamSynchronize(s);
+        return false;
+    }
+    DerivPArgs pa{S, K, nI, tp.dItems, tp.dItems + N, tp.dRate, tp.dQ, tp.PT64, tp.P1T, tp.P2T};
+    if (!hipOk(launchPderiv(pa, s), "launch P derivative kernel")) {
+        (void)hipStreamSynchronize(s);
+        return false;
+    }
+    DerivArgs a{};
+    a.dim = S;
+    a.K = K;
+    a.nPat = tp.nPat;
+    a.nPatPad = tp.nPatPad;
+    a.nOps = nOps;
+    a.waveLayout = tp.wave ? 1 : 0;
+    a.ops = dOps;
+    a.sibs = dSibs;
+    a.tips = dp->dev.tips;
+    a.ambMask = dp->dev.ambMask;
+    a.cl = reinterpret_cast<const double *>(tp.cl);
+    a.cl2 = tp.cl2;
+    a.PT64 = tp.PT64;
+    a.P1T = tp.P1T;
+    a.P2T = tp.P2T;
+    a.rootPi = reinterpret_cast<const double *>(tp.dArena + tp.offPi);   // staged by the evaluation just run
+    a.rootLeafRow = t->nodes[t->root]->isLeaf ? t->nodes[t->root]->seqNum : -1;
+    a.pInvar = mp->pInvar;
+    a.counts = dp->dev.counts;
+    a.invMask = dp->dev.invMask;
+    a.nWords = dp->dev.nWords;
+    a.partial = tp.dPartial;
+    a.out = tp.dOut;
+    if (!hipOk(launchEdgeDerivs(a, nBlocks, s), "launch derivative kernels")) {
+        (void)hipStreamSynchronize(s);
+        return false;
+    }
+    out.assign((size_t)3 * N, 0.0);
+    if (!hipOk(hipMemcpyAsync(out.data(), tp.dOut, (size_t)3 * N * 8, hipMemcpyDeviceToHost, s), "copy derivatives") ||
+        !hipOk(hipStreamSynchronize(s), "derivative kernels"))
+        return false;
+    for (int k = 0; k < 3; k++) out[(size_t)t->root * 3 + k] = 0.0;
+    tp.derivValid = true;
     return true;
 }
 
@@ -2371,6 +2583,273 @@ int p4g_getCondLikes(const p4g_node *n, int pNum, double *out)
                 if (!e.empty()) x = std::ldexp(x, -e[tp.wave ? (size_t)k * tp.nPatPad + p : (size_t)p]);
                 out[((size_t)k * S + s) * nP + p] = x;
             }
+    return 0;
+}
+
+// ------------------------------------------- branch-length derivatives
+int p4g_branchDerivatives(p4g_tree *t, int pNum, double *lnLEdge, double *d1, double *d2)
+{
+    clearError();
+    if (!t || !lnLEdge || !d1 || !d2 || pNum >= t->nParts) {
+        setError("branchDerivatives: bad arguments");
+        return 1;
+    }
+    if (!treeReady(t)) return 1;
+    if (t->precision != FP64) {
+        setError("branchDerivatives: the tree's partials must be FP64 (setPrecision 0)");
+        return 1;
+    }
+    const int N = t->nNodes;
+    for (int n = 0; n < N; n++) lnLEdge[n] = d1[n] = d2[n] = 0.0;
+    std::vector<double> out;
+    double sum = 0.0;
+    for (int p = 0; p < t->nParts; p++) {
+        if (pNum >= 0 && p != pNum) continue;
+        double v = 0.0;
+        if (!evalPart(t, p, true, false, &v)) return 1;
+        sum += v;
+        if (!derivPart(t, p, out)) return 1;
+        for (int n = 0; n < N; n++) {
+            lnLEdge[n] += out[(size_t)n * 3];
+            d1[n] += out[(size_t)n * 3 + 1];
+            d2[n] += out[(size_t)n * 3 + 2];
+        }
+    }
+    if (pNum < 0) t->logLike = sum;
+    return 0;
+}
+
+// Up-pass partials of a non-root node as [nCat][dim][nPatterns] (the
+// reference cl2 layout, Pf/p4_node.c:858-1018), after p4g_branchDerivatives.
+int p4g_getCondLikes2(const p4g_node *n, int pNum, double *out)
+{
+    clearError();
+    if (!n || !out || pNum < 0 || pNum >= n->tree->nParts) return 1;
+    const Tree *t = n->tree;
+    const TreePart &tp = t->tp[pNum];
+    if (!tp.derivValid || n->nodeNum == t->root) {
+        setError("getCondLikes2: no up-pass partials (call branchDerivatives; the root has none)");
+        return 1;
+    }
+    const int K = tp.K, S = tp.dim, nP = tp.nPat;
+    const size_t per = (size_t)K * tp.nPatPad * S;
+    std::vector<double> v(per);
+    if (!hipOk(hipStreamSynchronize(stream()), "sync") ||
+        !hipOk(hipMemcpy(v.data(), tp.cl2 + (size_t)n->nodeNum * per, per * 8, hipMemcpyDeviceToHost), "copy cl2"))
+        return 1;
+    for (int k = 0; k < K; k++)
+        for (int s = 0; s < S; s++)
+            for (int p = 0; p < nP; p++) out[((size_t)k * S + s) * nP + p] = v[((size_t)k * tp.nPatPad + p) * S + s];
+    return 0;
+}
+
+static int getPDeriv(const p4g_node *n, int pNum, double *out, int which)
+{
+    clearError();
+    if (!n || !out || pNum < 0 || pNum >= n->tree->nParts) return 1;
+    const Tree *t = n->tree;
+    const TreePart &tp = t->tp[pNum];
+    if (!tp.derivValid || n->nodeNum == t->root) {
+        setError("getBigPDeck derivative: none stored (call branchDerivatives; the root has none)");
+        return 1;
+    }
+    const int K = tp.K, S = tp.dim;
+    std::vector<double> v((size_t)K * S * S);
+    const double *src = (which == 1 ? tp.P1T : tp.P2T) + (size_t)n->nodeNum * K * S * S;
+    if (!hipOk(hipStreamSynchronize(stream()), "sync") ||
+        !hipOk(hipMemcpy(v.data(), src, v.size() * 8, hipMemcpyDeviceToHost), "copy P derivative"))
+        return 1;
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < S; i++)
+            for (int j = 0; j < S; j++) out[((size_t)k * S + i) * S + j] = v[((size_t)k * S + j) * S + i];
+    return 0;
+}
+
+int p4g_getBigPDeck1stD(const p4g_node *n, int pNum, double *out) { return getPDeriv(n, pNum, out, 1); }
+int p4g_getBigPDeck2ndD(const p4g_node *n, int pNum, double *out) { return getPDeriv(n, pNum, out, 2); }
+
+// ------------------------------------------------------------ CPO / LPML
+int p4g_cpoAccumulate(p4g_tree *t, double *lnL)
+{
+    clearError();
+    if (!t || !lnL) {
+        setError("cpoAccumulate: bad arguments");
+        return 1;
+    }
+    if (!treeReady(t)) return 1;
+    hipStream_t s = stream();
+    for (int p = 0; p < t->nParts; p++) {
+        Part *dp = t->data->parts[p];
+        if (dp->cpoSum && dp->cpoPatterns != dp->nPatterns) {
+            setError("cpoAccumulate: part " + std::to_string(p) + " changed its patterns since its CPO sums began; "
+                     "call p4g_cpoReset");
+            return 1;
+        }
+        if (!dp->cpoSum) {
+            if (!devAlloc(&dp->cpoSum, (size_t)dp->nPatterns, "hipMalloc CPO sums") ||
+                !hipOk(launchFillU32(reinterpret_cast<unsigned *>(dp->cpoSum), 0u, 2 * dp->nPatterns, s),
+                       "clear CPO sums"))
+                return 1;
+            dp->cpoPatterns = dp->nPatterns;
+            dp->cpoSamples = 0;
+        }
+    }
+    double sum = 0.0;
+    bool bad = false;
+    for (int p = 0; p < t->nParts; p++) {
+        double v = 0.0;
+        if (!evalPart(t, p, true, false, &v, nullptr, t->data->parts[p]->cpoSum)) return 1;
+        bad |= t->tp[p].hResult[1] != 0.0;
+        sum += v;
+    }
+    for (int p = 0; p < t->nParts; p++) t->data->parts[p]->cpoSamples++;
+    t->logLike = sum;
+    *lnL = sum;
+    if (bad) {
+        setError("cpoAccumulate: a site likelihood is zero, so its inverse (and the CPO sums) are infinite");
+        return 1;
+    }
+    return 0;
+}
+
+int p4g_cpoGetSums(const p4g_part *p, double *out, int *nSamples)
+{
+    clearError();
+    if (!p || !out || !nSamples) {
+        setError("cpoGetSums: bad arguments");
+        return 1;
+    }
+    *nSamples = p->cpoSamples;
+    const int nSites = ((int)p->seqPosPatternIndex.size() == p->nChar && p->nChar > 0) ? p->nChar : p->nPatterns;
+    if (!p->cpoSum) {
+        std::fill(out, out + nSites, 0.0);
+        return 0;
+    }
+    std::vector<double> h(p->cpoPatterns);
+    if (!hipOk(hipStreamSynchronize(stream()), "sync") ||
+        !hipOk(hipMemcpy(h.data(), p->cpoSum, h.size() * 8, hipMemcpyDeviceToHost), "copy CPO sums"))
+        return 1;
+    if (nSites == p->nChar && (int)p->seqPosPatternIndex.size() == p->nChar)
+        for (int i = 0; i < nSites; i++) out[i] = h[p->seqPosPatternIndex[i]];
+    else
+        std::copy(h.begin(), h.end(), out);
+    return 0;
+}
+
+int p4g_cpoReset(p4g_part *p)
+{
+    clearError();
+    if (!p) return 1;
+    if (p->cpoSum) {
+        if (!hipOk(hipStreamSynchronize(stream()), "sync")) return 1;
+        devFree(p->cpoSum);
+    }
+    p->cpoPatterns = 0;
+    p->cpoSamples = 0;
+    return 0;
+}
+
+int p4g_partNSites(const p4g_part *p)
+{
+    if (!p) return -1;
+    return ((int)p->seqPosPatternIndex.size() == p->nChar && p->nChar > 0) ? p->nChar : p->nPatterns;
+}
+
+void p4g_philox4x64(const unsigned long long *counter, const unsigned long long *key, unsigned long long *out)
+{
+    Philox4 c{{counter[0], counter[1], counter[2], counter[3]}};
+    const Philox4 r = philox4x64(c, key[0], key[1]);
+    for (int i = 0; i < 4; i++) out[i] = r.v[i];
+}
+
+// ------------------------------------------------------ ancestral draws
+int p4g_drawAncStates(p4g_tree *t, int pNum, int first, int n, const double *u, unsigned long long seed, int *draw)
+{
+    clearError();
+    if (!t || !draw || pNum < 0 || pNum >= t->nParts || first < 0 || n < 0) {
+        setError("drawAncStates: bad arguments");
+        return 1;
+    }
+    if (n == 0) return 0;
+    if (!treeReady(t)) return 1;
+    Part *dp = t->data->parts[pNum];
+    TreePart &tp = t->tp[pNum];
+    ModelPart *mp = t->model->parts[pNum];
+    if ((long long)first + n > dp->dev.nSites) {
+        setError("drawAncStates: sites [" + std::to_string(first) + ", " + std::to_string(first + n) +
+                 ") run past the part's " + std::to_string(dp->dev.nSites) + " sites");
+        return 1;
+    }
+    if (tp.K > ANC_MAX_K) {
+        setError("drawAncStates: at most 16 gamma categories");
+        return 1;
+    }
+    // bring the partials up to date (Tree.ancestralStateDraw evaluates first)
+    double v = 0.0;
+    if (!evalPart(t, pNum, !tp.everComputed, false, &v)) return 1;
+    std::vector<DerivSib> kids;
+    for (int c = t->nodes[t->root]->leftChild; c >= 0; c = t->nodes[c]->sibling) {
+        const Node *cn = t->nodes[c];
+        kids.push_back(DerivSib{cn->isLeaf ? 0 : 1, cn->isLeaf ? cn->seqNum : t->slotOf[c], c, 0});
+    }
+    const size_t kidBytes = (std::max<size_t>(kids.size(), 1) * sizeof(DerivSib) + 255) & ~(size_t)255;
+    const size_t uBytes = u ? (((size_t)n * 8 + 255) & ~(size_t)255) : 0;
+    const size_t need = kidBytes + uBytes + (size_t)n * 16;
+    hipStream_t s = stream();
+    if (need > tp.ancCap) {
+        if (tp.ancBuf) {
+            if (!hipOk(hipStreamSynchronize(s), "sync")) return 1;
+            (void)hipFree(tp.ancBuf);
+            tp.ancBuf = nullptr;
+            tp.ancCap = 0;
+        }
+        if (!hipOk(hipMalloc(&tp.ancBuf, need), "hipMalloc ancestral draw buffer")) return 1;
+        tp.ancCap = need;
+    }
+    DerivSib *dKids = reinterpret_cast<DerivSib *>(tp.ancBuf);
+    double *dU = u ? reinterpret_cast<double *>(tp.ancBuf + kidBytes) : nullptr;
+    int *dDraw = reinterpret_cast<int *>(tp.ancBuf + kidBytes + uBytes);
+    AncArgs a{};
+    a.dim = tp.dim;
+    a.K = tp.K;
+    a.nPatPad = tp.nPatPad;
+    a.layout = tp.wave ? CL_WAVE : pruneUsesMfma(tp.dim, tp.K, t->precision) ? CL_TILED : CL_GROUPED;
+    a.fp32 = t->precision == FP32 ? 1 : 0;
+    a.cl = tp.cl;
+    a.slotBytes = tp.clBytes;
+    a.scl = tp.scl;
+    a.sclPerSlot = tp.sclPerSlot;
+    a.nKid = (int)kids.size();
+    a.kids = dKids;
+    a.tips = dp->dev.tips;
+    a.ambMask = dp->dev.ambMask;
+    a.PT64 = tp.PT64;
+    a.rootPi = reinterpret_cast<const double *>(tp.dArena + tp.offPi);   // staged by the evaluation above
+    a.pInvar = mp->pInvar;
+    a.invMask = dp->dev.invMask;
+    a.nWords = dp->dev.nWords;
+    a.sitePat = dp->dev.sitePat;
+    a.first = first;
+    a.n = n;
+    a.u = dU;
+    a.seed = (uint64_t)seed;
+    a.draw = dDraw;
+    if ((!kids.empty() &&
+         !hipOk(hipMemcpyAsync(dKids, kids.data(), kids.size() * sizeof(DerivSib), hipMemcpyHostToDevice, s),
+                "stage root children")) ||
+        (u && !hipOk(hipMemcpyAsync(dU, u, (size_t)n * 8, hipMemcpyHostToDevice, s), "stage uniforms")) ||
+        !hipOk(launchAncDraw(a, s), "launch ancestral draw kernel") ||
+        !hipOk(hipMemcpyAsync(draw, dDraw, (size_t)n * 16, hipMemcpyDeviceToHost, s), "copy draws") ||
+        !hipOk(hipStreamSynchronize(s), "ancestral draw kernel")) {
+        (void)hipStreamSynchronize(s);
+        return 1;
+    }
+    for (int i = 0; i < n; i++)
+        if (draw[4 * i + 1] < 0 && draw[4 * i + 2] == 0) {
+            setError("drawAncStates: site " + std::to_string(first + i) +
+                     " has a zero (or non-finite) root likelihood; nothing to draw from");
+            return 1;
+        }
     return 0;
 }
 

@@ -53,6 +53,8 @@ struct DevPart {
     double *counts = nullptr;     // [nPatPad] pattern weights (0 for padding)
     uint32_t *invMask = nullptr;  // [nWords][nPatPad] states compatible with every taxon
     uint64_t *ambMask = nullptr;  // [nAmb] state masks of partial ambiguities
+    int *sitePat = nullptr;       // [nSites] pattern of each site
+    int nSites = 0;
     int nAmb = 0;
     int nWords = 0;
     std::vector<uint64_t> hostAmb;
@@ -73,6 +75,9 @@ struct Part {
     bool invarSet = false;
     std::vector<double> siteLikes;  // [nChar]
     DevPart dev;
+    // CPO: per pattern, the sum over samples of 1/like, in GPU memory
+    double *cpoSum = nullptr;
+    int cpoPatterns = 0, cpoSamples = 0;
     ~Part();
 };
 
@@ -271,6 +276,16 @@ struct TreePart {
     double rootPiUp[MAX_DIM] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double lastKernelUs = -1.0;
+    // branch-length derivatives (allocated by their first use)
+    double *cl2 = nullptr, *P1T = nullptr, *P2T = nullptr, *dQ = nullptr, *dRate = nullptr;
+    double *dPartial = nullptr, *dOut = nullptr;
+    int *dItems = nullptr;
+    void *dProg = nullptr;
+    size_t dPartialCap = 0, dProgCap = 0;
+    bool derivValid = false;   // cl2, P1T and P2T hold the last derivative call's values
+    // ancestral draws: the root's children, uniforms and draws
+    char *ancBuf = nullptr;
+    size_t ancCap = 0;
 };
 
 struct Tree {
@@ -351,6 +366,49 @@ struct PruneArgs {
     int nAmb;                // entries of ambMask
 };
 
+// ---------------------------------------------- branch-length derivatives
+// One entry per non-root node of the up pass (preorder), and the node's
+// siblings (the other children of its parent).
+constexpr int DOP_LEAF = 1;          // the node is a leaf (idx: its tip row)
+constexpr int DOP_PARENT_ROOT = 2;   // its parent is the root
+struct DerivOp {
+    int node, flags, idx, parent, sibStart, nSib, pad0, pad1;   // idx: tip row or partials slot
+};
+struct DerivSib {
+    int kind, idx, pnode, pad;   // kind 0 tip (idx: tip row), 1 internal (idx: partials slot)
+};
+constexpr int DERIV_MAX_K = 16;
+struct DerivPArgs {
+    int dim, K, nItems;
+    const int *nodeList;         // [nItems]
+    const int *eigSlot;          // [nItems] (comp, rMatrix) slot of each node
+    const double *rateFac;       // [nItems][K]: d(effective length)/d(branch length)
+    const double *Q;             // [nEig][dim][dim] normalised rate matrices
+    const double *PT64;          // P^T of the tree part
+    double *P1T, *P2T;           // P'^T and P''^T, same layout
+};
+struct DerivArgs {
+    int dim, K, nPat, nPatPad, nOps;
+    int waveLayout;              // the down-pass partials are [slot][cat][pattern][4]
+    const DerivOp *ops;
+    const DerivSib *sibs;
+    const uint8_t *tips;
+    const uint64_t *ambMask;
+    const double *cl;            // FP64 down-pass partials
+    double *cl2;                 // [nNodes][K][nPatPad][dim]
+    const double *PT64, *P1T, *P2T;
+    const double *rootPi;
+    int rootLeafRow;
+    double pInvar;
+    const double *counts;
+    const uint32_t *invMask;
+    int nWords;
+    double *partial;             // [nOps][nBlocks][3]
+    double *out;                 // [nNodes][3]: lnL across the node's branch, d1, d2
+};
+hipError_t launchPderiv(const DerivPArgs &a, hipStream_t s);
+hipError_t launchEdgeDerivs(const DerivArgs &a, int nBlocks, hipStream_t s);
+
 // Kernel launchers (p4g_kernels.hip).
 hipError_t launchEig(const EigArgs &a, hipStream_t s);
 hipError_t launchPmat(const PmatArgs &a, hipStream_t s);
@@ -407,6 +465,71 @@ hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, 
 // n words of p set to v by a small kernel (usable under stream capture,
 // unlike hipMemsetAsync)
 hipError_t launchFillU32(unsigned *p, unsigned v, int n, hipStream_t s);
+
+// ------------------------------------------- root-side consumers (p4g_anc.hip)
+// Philox4x64-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers:
+// as easy as 1, 2, 3", SC 2011): four 64-bit words from a 256-bit counter
+// and a 128-bit key.  Counter-based, so each (stream, site, draw) has its
+// own numbers whatever the launch shape.
+struct Philox4 {
+    uint64_t v[4];
+};
+__host__ __device__ inline Philox4 philox4x64(Philox4 c, uint64_t k0, uint64_t k1)
+{
+    const uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+    const uint64_t W0 = 0x9E3779B97F4A7C15ull, W1 = 0xBB67AE8584CAA73Bull;
+    for (int r = 0; r < 10; r++) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint64_t hi0 = __umul64hi(M0, c.v[0]), hi1 = __umul64hi(M1, c.v[2]);
+#else
+        const uint64_t hi0 = (uint64_t)(((unsigned __int128)M0 * c.v[0]) >> 64);
+        const uint64_t hi1 = (uint64_t)(((unsigned __int128)M1 * c.v[2]) >> 64);
+#endif
+        const uint64_t lo0 = M0 * c.v[0], lo1 = M1 * c.v[2];
+        Philox4 n;
+        n.v[0] = hi1 ^ c.v[1] ^ k0;
+        n.v[1] = lo1;
+        n.v[2] = hi0 ^ c.v[3] ^ k1;
+        n.v[3] = lo0;
+        c = n;
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+// The top 53 bits as a double in [0, 1).
+__host__ __device__ inline double uniform53(uint64_t x) { return (double)(x >> 11) * 0x1.0p-53; }
+
+constexpr int ANC_MAX_K = 16;
+// Where the partials of a slot live (the layouts of the pruning kernels).
+constexpr int CL_GROUPED = 0;   // [slot][pattern][cat][state], exponents [slot][pattern]
+constexpr int CL_WAVE = 1;      // [slot][cat][pattern][4],     exponents [slot][cat][pattern]
+constexpr int CL_TILED = 2;     // 20 states: [slot][pattern/16][cat][state/4][state%4][pattern%16],
+                                // exponents [slot][pattern]
+struct AncArgs {
+    int dim, K, nPatPad, layout, fp32;
+    const void *cl;              // the tree part's partials
+    size_t slotBytes;
+    const int *scl;              // FP32 exponents (fp32 only)
+    size_t sclPerSlot;
+    int nKid;
+    const DerivSib *kids;        // the root's children: tip row or partials slot, P node
+    const uint8_t *tips;
+    const uint64_t *ambMask;
+    const double *PT64;
+    const double *rootPi;
+    double pInvar;
+    const uint32_t *invMask;
+    int nWords;
+    const int *sitePat;          // [nChar] pattern of each site
+    int first, n;                // sites [first, first + n)
+    const double *u;             // [n] uniforms, or null: Philox stream (seed, site)
+    uint64_t seed;
+    int *draw;                   // [n][4] chStNum, catNum, isInvar, invarChNum
+};
+hipError_t launchAncDraw(const AncArgs &a, hipStream_t s);
+// sum[p] += 1 / like[p] for p < nPat (the CPO sums)
+hipError_t launchCpoAccum(const double *like, double *sum, int nPat, hipStream_t s);
 
 // Host numerics (p4g_host.hip).
 int discreteGamma(double *freqK, double *rK, double alfa, double beta, int K, int median);

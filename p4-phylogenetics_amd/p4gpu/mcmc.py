@@ -428,6 +428,13 @@ class Mcmc(object):
         # True: the likelihood ratio is left out of the acceptance ratio, so
         # the chain samples the prior (a check of the proposal ratios).
         self.samplePrior = False
+        # CPO (p4/mcmc.py:918-921): with doCpo, every sample after
+        # cpo_startGen adds 1/siteLike of the cold chain's tree to per-site
+        # sums held in GPU memory by the data parts.
+        self.doCpo = False
+        self.cpo_startGen = 0
+        for dp in aTree.data.parts:
+            pf.cpoReset(dp.cPart)
 
     def addLearnedProposal(self, learned, weight):
         """Add the learned joint branch-length move (p4gpu.learned)."""
@@ -521,8 +528,37 @@ class Mcmc(object):
                 pr.append(float(g.val[0]))
             pr.append(mp.pInvar.val)
         self.samples['prams'].append(pr)
+        if self.doCpo and self.gen + 1 > self.cpo_startGen:
+            pf.cpoAccumulate(t.cTree)
         for f in self.sampleHooks:
             f(t)
+
+    # ------------------------------------------------------------- CPO
+    @property
+    def cpo_nSamples(self):
+        return pf.cpoGetSums(self.tree.data.parts[0].cPart)[1]
+
+    @property
+    def cpo_sumsOfInverseSiteLikes(self):
+        """Per site, all parts in order: the sum over CPO samples of
+        1/siteLike (p4/mcmc.py:3292-3296)."""
+        return numpy.concatenate([pf.cpoGetSums(dp.cPart)[0] for dp in self.tree.data.parts])
+
+    def lpml(self, verbose=True):
+        """Log pseudo-marginal likelihood from the CPO sums
+        (p4/mcmc.py:3730-3748): sum over sites of -log(mean of 1/siteLike)."""
+        if not self.doCpo:
+            print("Mcmc.lpml().  doCpo was not on.")
+            return 0.0
+        n = self.cpo_nSamples
+        if not n:
+            print("Mcmc.lpml().  no cpo_nSamples?")
+            return 0.0
+        a = self.cpo_sumsOfInverseSiteLikes / n
+        v = float((-numpy.log(a)).sum())
+        if verbose:
+            print(f"cpo_nSamples: {n}, lpml: {v}")
+        return v
 
     # ------------------------------------------------------------ judges
     def splitFrequencies(self, burnin=0):

@@ -136,6 +136,16 @@ def _declare():
     _sig("p4g_verifyIdentityOfTwoTrees", I, V, V)
     _sig("p4g_getBigPDeck", I, V, I, V)
     _sig("p4g_getCondLikes", I, V, I, V)
+    _sig("p4g_branchDerivatives", I, V, I, V, V, V)
+    _sig("p4g_getCondLikes2", I, V, I, V)
+    _sig("p4g_getBigPDeck1stD", I, V, I, V)
+    _sig("p4g_getBigPDeck2ndD", I, V, I, V)
+    _sig("p4g_cpoAccumulate", I, V, V)
+    _sig("p4g_cpoGetSums", I, V, V, V)
+    _sig("p4g_cpoReset", I, V)
+    _sig("p4g_partNSites", I, V)
+    _sig("p4g_drawAncStates", I, V, I, I, I, V, ctypes.c_ulonglong, V)
+    _sig("p4g_philox4x64", None, V, V, V)
     _sig("p4g_newBatch", V, I, V)
     _sig("p4g_mlpNew", V, I, I, I, ctypes.c_ulonglong)
     _sig("p4g_mlpFree", None, V)
@@ -472,6 +482,100 @@ def p4_getBigPDeck(cNode, pNum, nCat, dim):
     out = numpy.zeros((nCat, dim, dim), numpy.float64)
     _ok(lib().p4g_getBigPDeck(cNode, pNum, out.ctypes.data), "p4_getBigPDeck")
     return out
+
+
+def branchDerivatives(cTree, pNum, nNodes):
+    """(lnLEdge, d1, d2), arrays indexed by node number: the log likelihood
+    across each non-root node's branch and its first and second derivatives
+    with respect to that branch length (part pNum, or all parts for -1).
+    What Pf's Newton optimiser computes one node at a time
+    (Pf/p4_treeNewt.c:210-596); FP64 trees only."""
+    lnl = numpy.zeros(nNodes)
+    d1 = numpy.zeros(nNodes)
+    d2 = numpy.zeros(nNodes)
+    _ok(lib().p4g_branchDerivatives(cTree, pNum, lnl.ctypes.data, d1.ctypes.data, d2.ctypes.data),
+        "branchDerivatives")
+    return lnl, d1, d2
+
+
+def p4_getCondLikes2(cNode, pNum, nCat, dim, nPatterns):
+    out = numpy.zeros((nCat, dim, nPatterns), numpy.float64)
+    _ok(lib().p4g_getCondLikes2(cNode, pNum, out.ctypes.data), "p4_getCondLikes2")
+    return out
+
+
+def p4_getBigPDeck1stD(cNode, pNum, nCat, dim):
+    out = numpy.zeros((nCat, dim, dim), numpy.float64)
+    _ok(lib().p4g_getBigPDeck1stD(cNode, pNum, out.ctypes.data), "p4_getBigPDeck1stD")
+    return out
+
+
+def p4_getBigPDeck2ndD(cNode, pNum, nCat, dim):
+    out = numpy.zeros((nCat, dim, dim), numpy.float64)
+    _ok(lib().p4g_getBigPDeck2ndD(cNode, pNum, out.ctypes.data), "p4_getBigPDeck2ndD")
+    return out
+
+
+# ------------------------------------------------------------- CPO / LPML
+def cpoAccumulate(cTree):
+    """One CPO sample of tree cTree (p4/mcmc.py:3286-3297 without the host
+    round trip): every part's per-pattern sum of 1/like grows in GPU
+    memory.  Returns the tree's log likelihood."""
+    out = ctypes.c_double()
+    _ok(lib().p4g_cpoAccumulate(cTree, ctypes.byref(out)), "cpoAccumulate")
+    return out.value
+
+
+def cpoGetSums(cPart):
+    """(per-site sums of 1/siteLike, number of samples) of one part."""
+    n = lib().p4g_partNSites(cPart)
+    out = numpy.zeros(max(n, 0), numpy.float64)
+    ns = ctypes.c_int()
+    _ok(lib().p4g_cpoGetSums(cPart, out.ctypes.data, ctypes.byref(ns)), "cpoGetSums")
+    return out, ns.value
+
+
+def cpoReset(cPart):
+    _ok(lib().p4g_cpoReset(cPart), "cpoReset")
+
+
+def partNSites(cPart):
+    return lib().p4g_partNSites(cPart)
+
+
+# ------------------------------------------------------- ancestral states
+def drawAncStates(cTree, pNum, first, n, u=None, seed=0):
+    """Root ancestral-state draws for sites [first, first + n) of part pNum:
+    an int32 array [n][4] of (chStNum, catNum, isInvar, invarChNum) as
+    pf.p4_drawAncState fills for one site.  u: n uniforms in [0, 1), or
+    None for the GPU's Philox stream keyed by (seed, site)."""
+    draw = numpy.zeros((n, 4), numpy.int32)
+    if u is not None:
+        u = numpy.ascontiguousarray(u, numpy.float64)
+        if u.shape != (n,):
+            raise P4Error("drawAncStates: u must hold n values")
+    _ok(lib().p4g_drawAncStates(cTree, pNum, first, n, None if u is None else u.ctypes.data,
+                                ctypes.c_ulonglong(seed & 0xFFFFFFFFFFFFFFFF), draw.ctypes.data),
+        "drawAncStates")
+    return draw
+
+
+def philox4x64(counter, key):
+    """Philox4x64-10 (the GPU draws' generator) of counter[4], key[2]."""
+    c = numpy.ascontiguousarray(counter, numpy.uint64)
+    k = numpy.ascontiguousarray(key, numpy.uint64)
+    out = numpy.zeros(4, numpy.uint64)
+    lib().p4g_philox4x64(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def p4_drawAncState(cTree, partNum, seqPos, draw):
+    """pf.p4_drawAncState (Pf/pfmodule.c:2353-2367): one site's draw into
+    the int32 array draw[4], with a fresh 64-bit seed from Python's
+    random module."""
+    import random
+    d = drawAncStates(cTree, partNum, seqPos, 1, seed=random.getrandbits(64))
+    draw[:4] = d[0]
 
 
 def p4_getCondLikes(cNode, pNum, nCat, dim, nPatterns):

@@ -555,6 +555,84 @@ class Tree(object):
             out += pf.getSiteLikes(p.cPart)
         return out
 
+    def ancestralStateDraw(self, seed=None):
+        """One draw of the root's character states, all sites, as a string
+        (p4/tree.py:9640-9676): for each site a (gamma category, state) or
+        an invariant state is drawn from the root's conditional likelihoods
+        and the root composition, and the state's symbol is kept.  All
+        sites of a part are drawn in one GPU launch; seed (default: from
+        Python's random module) keys the GPU's Philox stream."""
+        self._commonCStuff()
+        self.logLike = pf.p4_treeLogLike(self.cTree, 0)
+        out = []
+        for pNum, dp in enumerate(self._data.parts):
+            n = pf.partNSites(dp.cPart)
+            s = random.getrandbits(64) if seed is None else (int(seed) + pNum) & 0xFFFFFFFFFFFFFFFF
+            d = pf.drawAncStates(self.cTree, pNum, 0, n, seed=s)
+            st = numpy.where(d[:, 1] >= 0, d[:, 0], d[:, 3])
+            if (st < 0).any():
+                raise P4Error("Tree.ancestralStateDraw(): a site got no draw")
+            sym = numpy.frombuffer(dp.symbols.encode(), numpy.uint8)
+            out.append(sym[st].tobytes().decode())
+        return ''.join(out)
+
+    def branchDerivatives(self, pNum=-1):
+        """(lnLEdge, d1, d2) by node number: the log likelihood computed
+        across each non-root node's branch (it equals the tree's) and its
+        first and second derivatives with respect to that branch length,
+        for part pNum or summed over all parts (-1).  FP64 only."""
+        self._commonCStuff()
+        return pf.branchDerivatives(self.cTree, pNum, len(self.nodes))
+
+    def optBrLensNewton(self, epsilon=1.0e-5, likeDelta=1.0e-7, maxIter=100, verbose=0):
+        """Maximise the likelihood over all branch lengths (model fixed) by
+        Newton-Raphson on every branch at once, from the derivatives of
+        branchDerivatives().  Each branch's step follows p4_newtNode
+        (Pf/p4_treeNewt.c:543-590): t - d1/d2, or t/5 where the curvature
+        is not negative, clipped to [BRLEN_MIN, BRLEN_MAX] and to five times
+        the length before the step.  The reference moves one branch at a time; here
+        all move together, and a step that lowers the likelihood is halved
+        until it does not.  Stops when every |d1| < epsilon, or when a full
+        (unhalved) step gains less than likeDelta.  Returns the final log
+        likelihood."""
+        lo, hi = var.BRLEN_MIN, var.BRLEN_MAX
+        nodes = [n for n in self.iterNodesNoRoot()]
+        lnl, d1, d2 = self.branchDerivatives()
+        cur = float(lnl[nodes[0].nodeNum])
+        for it in range(maxIter):
+            t = numpy.array([n.br.len for n in nodes])
+            g1 = numpy.array([d1[n.nodeNum] for n in nodes])
+            g2 = numpy.array([d2[n.nodeNum] for n in nodes])
+            free = ~(((t <= lo) & (g1 < 0)) | ((t >= hi) & (g1 > 0)))
+            if not (numpy.abs(g1[free]) >= epsilon).any():
+                break
+            with numpy.errstate(divide='ignore', invalid='ignore'):
+                nt = numpy.where(g2 < 0.0, t - g1 / g2, t / 5.0)
+            nt = numpy.clip(numpy.minimum(nt, 5.0 * t), lo, hi)
+            step = nt - t
+            halved = 0
+            for halved in range(30):
+                for n, v in zip(nodes, t + step):
+                    n.br.len = float(v)
+                new = self.calcLogLike(verbose=0)
+                if new >= cur - 1e-12 * abs(cur):
+                    break
+                step *= 0.5
+            else:
+                for n, v in zip(nodes, t):
+                    n.br.len = float(v)
+                new = self.calcLogLike(verbose=0)
+                break
+            gain = new - cur
+            cur = new
+            if verbose:
+                print("optBrLensNewton %3d  lnL %.8f  gain %.3g" % (it, cur, gain))
+            lnl, d1, d2 = self.branchDerivatives()
+            if halved == 0 and abs(gain) < likeDelta:
+                break
+        self.logLike = self.calcLogLike(verbose=0)
+        return self.logLike
+
     def setPrecision(self, precision):
         """'fp64' (default, the reference's numerics) or 'fp32' (FP32
         partials with power-of-two scale exponents)."""

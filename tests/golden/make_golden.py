@@ -155,6 +155,33 @@ def pmat_vectors():
     print("pmat                   4- and 20-state, %d branch lengths" % len(ts))
 
 
+DERIV_CASES = ["gtr16_500", "het64_800", "prot128_150", "edge_dna_gaps_pinv", "multifurcation_root4"]
+
+
+def deriv_vectors():
+    """Branch-length derivatives of the likelihood fixtures through the
+    reference's own P', P'' and up-pass conditional likelihood code
+    (Pf/p4_node.c:442-540, 858-1018), with the per-site sums of
+    Pf/p4_treeNewt.c:257-500 restated in oracle/ref_driver.c: per non-root
+    node the lnL computed across its branch and d lnL / dt, d2 lnL / dt2;
+    P' and P'' of the first nodes; the up-pass partials of a few nodes."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import load, oracle_flat
+    for case in DERIV_CASES:
+        fx = load(case)
+        f = oracle_flat(fx)
+        N = len(fx["parent"])
+        root = int(fx["root"])
+        internal = [n for n in range(N) if not fx["isLeaf"][n] and n != root]
+        leaves = [n for n in range(N) if fx["isLeaf"][n] and n != root]
+        want = sorted(set(internal[:2] + internal[-1:] + leaves[:2]))
+        lnl, d1, d2, P1, P2, cl2 = oracle.ref_part_derivs(f, want_cl2=want)
+        numpy.savez_compressed(os.path.join(HERE, "derivs_" + case + ".npz"), lnLEdge=lnl, d1=d1, d2=d2,
+                               P1=P1[:min(6, N)], P2=P2[:min(6, N)], cl2Nodes=numpy.array(want, numpy.int32),
+                               cl2=numpy.array([cl2[n] for n in want]))
+        print("derivs_%-16s d1[1] %.10g  d2[1] %.10g" % (case, d1[1], d2[1]))
+
+
 if __name__ == "__main__":
     if not oracle.ref_available():
         sys.exit("oracle/_ref/libp4ref.so is missing: run make -C oracle where the reference is present")
@@ -166,3 +193,4 @@ if __name__ == "__main__":
     edge_dna_case()
     multifurcation_case()
     leafroot_case()
+    deriv_vectors()

@@ -39,9 +39,9 @@ def oracle_flat(fx):
     import oracle
     f = flat_from_fixture(fx)
     iv, ia = oracle.invar_sites(f.patterns, f.dim, f.equates)
-    f.invarVec, f.invarArray = iv, ia
-    f._build()
-    return f
+    kw = dict(f.kwargs)
+    kw.update(invarVec=iv, invarArray=ia)
+    return oracle.Flat(**kw)
 
 
 def tree_from_fixture(fx, precision="fp64"):
