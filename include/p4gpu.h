@@ -70,6 +70,7 @@ int p4g_setGlobalInvarSitesVec(p4g_part *p);
  * globalInvarSitesVec/Array; Pf/pftypes.h:31-53). */
 int p4g_partNPatterns(const p4g_part *p);
 int p4g_partNChar(const p4g_part *p);
+int p4g_partNTax(const p4g_part *p);
 int p4g_partDim(const p4g_part *p);
 /* Copies out: patterns [nTax][nPatterns], counts [nPatterns],
  * seqPosPatternIndex [nChar], invarVec [nPatterns], invarArray [dim][nPatterns].
@@ -271,6 +272,38 @@ int p4g_partNSites(const p4g_part *p);
  * 821-840).  The partials are brought up to date first. */
 int p4g_drawAncStates(p4g_tree *t, int pNum, int first, int n, const double *u, unsigned long long seed,
                       int *draw);
+/* ---- simulation and posterior-predictive summaries.
+ * p4g_simulate: pf.p4_simulate (Pf/p4_treeSim.c:14-590) -- new sequences
+ * for every part of t's data, one GPU thread per site: the site's gamma
+ * category (floor(nCat * u)), root state (running sums of the root comp,
+ * last entry 1.0) and invariant flag (u < pInvar), or with refTree those
+ * three from the root draws of refTree's data (p4g_drawAncStates); then
+ * each non-root node in preorder takes its parent's state through the
+ * running sums of its P row (the picker decks of Pf/p4_node.c:545-586);
+ * invariant sites keep the root state.  P comes from the current model and
+ * branch lengths.  Uniforms: Philox4x64-10 keyed by seed, counted by (site,
+ * node, part) -- the same draws for a seed whatever the launch shape (the
+ * reference's GSL stream is not reproduced).  Afterwards each part holds
+ * the leaves' sequences and no patterns, as after the reference's call
+ * (make them again with p4g_makePatterns). */
+int p4g_simulate(p4g_tree *t, p4g_tree *refTree, unsigned long long seed);
+/* pf.symbolSequences (Pf/part.c:604-680): the part's sequences as symbols,
+ * out [nTax * nChar] (no terminator). */
+int p4g_symbolSequences(const p4g_part *p, char *out);
+/* The last simulation's gamma category of each site, -1 where the site was
+ * invariant (part->simCats and globalInvarSitesVec of the reference). */
+int p4g_partSimCats(const p4g_part *p, int *cats);
+/* Sequence summaries on the GPU (Pf/part.c:1419-1571): compCounts
+ * [nTax][dim] per-taxon state counts, nConstant (partSimpleConstantSites-
+ * Count), meanNChars (partMeanNCharsPerSite), nNonState: codes that are
+ * gaps or ambiguities. */
+int p4g_partSeqStats(p4g_part *p, int *compCounts, int *nConstant, double *meanNChars, int *nNonState);
+/* pf.partBigXSquared (Pf/part.c:1490-1571): -2.0 if the part has gaps or
+ * ambiguities; -3.0 on an error. */
+double p4g_partBigXSquared(p4g_part *p);
+/* pf.getUnconstrainedLogLike (Pf/part.c:682-714) from the part's
+ * patterns. */
+double p4g_getUnconstrainedLogLike(const p4g_part *p);
 /* The counter-based generator of the GPU draws (no counterpart in the
  * reference, whose draws use random() and GSL): Philox4x64-10 of counter[4]
  * under key[2], on the host, from the same source as the device code. */

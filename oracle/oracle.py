@@ -431,3 +431,87 @@ def lpml(sums, nSamples):
     """p4/mcmc.py:3742-3746."""
     a = numpy.array(sums) / nSamples
     return float((-numpy.log(a)).sum())
+
+
+# ------------------------------------------------------------ simulation
+_M32 = numpy.uint64(0xFFFFFFFF)
+
+
+def _mulhilo64(a, b):
+    """(high, low) 64-bit words of the 128-bit products of uint64 arrays."""
+    a0, a1 = a & _M32, a >> numpy.uint64(32)
+    b0, b1 = b & _M32, b >> numpy.uint64(32)
+    p00, p01, p10, p11 = a0 * b0, a0 * b1, a1 * b0, a1 * b1
+    mid = (p00 >> numpy.uint64(32)) + (p01 & _M32) + (p10 & _M32)
+    hi = p11 + (p01 >> numpy.uint64(32)) + (p10 >> numpy.uint64(32)) + (mid >> numpy.uint64(32))
+    with numpy.errstate(over='ignore'):
+        lo = a * b
+    return hi, lo
+
+
+def philox4x64_vec(c0, c1, c2, c3, k0, k1):
+    """Philox4x64-10 (Salmon et al., SC 2011) over arrays of counters; the
+    scalar form is checked against numpy's Philox generator."""
+    M0, M1 = numpy.uint64(0xD2E7470EE14C6C93), numpy.uint64(0xCA5A826395121157)
+    W0, W1 = numpy.uint64(0x9E3779B97F4A7C15), numpy.uint64(0xBB67AE8584CAA73B)
+    c = [numpy.asarray(x, dtype=numpy.uint64) for x in (c0, c1, c2, c3)]
+    n = max(x.size for x in c)
+    c = [numpy.broadcast_to(x, (n,)).copy() for x in c]
+    k0, k1 = numpy.uint64(k0), numpy.uint64(k1)
+    with numpy.errstate(over='ignore'):
+        for _ in range(10):
+            hi0, lo0 = _mulhilo64(M0, c[0])
+            hi1, lo1 = _mulhilo64(M1, c[2])
+            c = [hi1 ^ c[1] ^ k0, lo1, hi0 ^ c[3] ^ k1, lo0]
+            k0 = k0 + W0
+            k1 = k1 + W1
+    return c
+
+
+def _u53(x):
+    return (x >> numpy.uint64(11)).astype(numpy.float64) * 2.0 ** -53
+
+
+SIM_CTR, SIM_KEY_ROOT, SIM_KEY_NODE = 0x73696d, 0x726f6f74, 0x6e6f6465
+
+
+def simulate(f, Pm, nChar, seed, pNum=0, ref_draw=None):
+    """Pf/p4_treeSim.c:14-590 for one part with the GPU's uniforms (Philox
+    counted by (site, node, part)): per site the category floor(K u), the
+    root state from the root comp's running sums (last entry 1.0) and the
+    invariant flag u < pInvar -- or those from ref_draw [nChar][4] -- then
+    each non-root node in preorder from the running sums of P[cat][parent
+    state] (the picker decks of Pf/p4_node.c:545-586).  Pm: [N][K][S][S]
+    P[s][j].  Returns (states [N][nChar], cats [nChar], -1 for invariant)."""
+    K, S, N = f.nCat, f.dim, len(f.parent)
+    root = int(f.root)
+    sites = numpy.arange(nChar, dtype=numpy.uint64)
+    states = numpy.zeros((N, nChar), numpy.int64)
+    if ref_draw is not None:
+        ref_draw = numpy.asarray(ref_draw)
+        inv = ref_draw[:, 2] != 0
+        cat = numpy.where(inv, 0, ref_draw[:, 1])
+        states[root] = numpy.where(inv, ref_draw[:, 3], ref_draw[:, 0])
+    else:
+        r = philox4x64_vec(sites, 0, pNum, SIM_CTR, seed, SIM_KEY_ROOT)
+        cat = numpy.minimum(numpy.floor(K * _u53(r[0])).astype(numpy.int64), K - 1) if K > 1 \
+            else numpy.zeros(nChar, numpy.int64)
+        comp = f.comps[f.compNums[root]]
+        picker = numpy.cumsum(comp)
+        picker[S - 1] = 1.0
+        u = _u53(r[1])
+        states[root] = numpy.argmax(u[:, None] < picker[None, :], axis=1)
+        inv = (_u53(r[2]) < f.pInvar) if f.pInvar > 0 else numpy.zeros(nChar, bool)
+    # preorder: the reverse of the postorder
+    post = [int(x) for x in f.postOrder if int(x) >= 0]
+    for n in reversed(post):
+        if n == root:
+            continue
+        ps = states[f.parent[n]]
+        u = _u53(philox4x64_vec(sites, n + 1, pNum, SIM_CTR, seed, SIM_KEY_NODE)[0])
+        deck = numpy.cumsum(Pm[n], axis=2)     # [K][S][S] running sums of each row
+        deck[:, :, S - 1] = 1.0
+        rows = deck[cat, ps]                   # [nChar][S]
+        new = numpy.argmax(u[:, None] < rows, axis=1)
+        states[n] = numpy.where(inv, ps, new)
+    return states, numpy.where(inv, -1, cat)

@@ -142,6 +142,7 @@ Part::~Part()
     devFree(dev.ambMask);
     devFree(dev.sitePat);
     devFree(cpoSum);
+    devFree(dSeq);
 }
 
 ModelPart::~ModelPart()
@@ -200,6 +201,8 @@ Tree::~Tree()
 {
     if (g_devState == 1) (void)hipStreamSynchronize(g_stream);
     for (auto &t : tp) freeTreePart(t);
+    if (simBuf) (void)hipFree(simBuf);
+    simBuf = nullptr;
 }
 
 // ------------------------------------------------------------- data side
@@ -371,15 +374,20 @@ static bool allocTreePart(Tree *t, int pNum)
     Part *dp = t->data->parts[pNum];
     if (!uploadPart(dp)) return false;
     const bool wave = pruneUsesWave(mp->dim, mp->nCat, t->precision);
-    if (tp.alloc && (tp.nPat != dp->nPatterns || tp.nPatPad != dp->dev.nPatPad)) {
-        setError("part " + std::to_string(pNum) +
-                 ": the data part's patterns changed after the tree's device state was made; free the tree "
-                 "and build it again");
-        return false;
-    }
-    if (tp.alloc && tp.precision == t->precision && tp.wave == wave) return true;
+    // the data part's patterns changed (a simulation, new sequences): every
+    // buffer sized by the pattern count is made again
+    const bool patsChanged = tp.alloc && (tp.nPat != dp->nPatterns || tp.nPatPad != dp->dev.nPatPad);
+    if (tp.alloc && !patsChanged && tp.precision == t->precision && tp.wave == wave) return true;
     if (tp.alloc) {
-        // precision or kernel change: the partial buffers are re-laid out
+        // precision, kernel or pattern change: the partial buffers are re-laid out
+        if (!hipOk(hipStreamSynchronize(stream()), "sync before re-layout")) return false;
+        if (patsChanged) {
+            devFree(tp.patLikes);
+            devFree(tp.cl2);
+            devFree(tp.dPartial);
+            tp.dPartialCap = 0;
+            tp.derivValid = false;
+        }
         if (tp.cl) (void)hipFree(tp.cl);
         tp.cl = nullptr;
         devFree(tp.scl);
@@ -473,7 +481,10 @@ static bool allocTreePart(Tree *t, int pNum)
     return true;
 }
 
-static bool treeReady(Tree *t)
+// needPatterns = false: a part whose tree state exists is left as it is
+// (parameters and simulation do not read the patterns, which a simulation
+// has cleared).
+static bool treeReady(Tree *t, bool needPatterns = true)
 {
     if (!deviceReady()) return false;
     if (!t->data || !t->model) {
@@ -498,7 +509,7 @@ static bool treeReady(Tree *t)
             setError("model or data part missing");
             return false;
         }
-        if (!allocTreePart(t, p)) return false;
+        if ((needPatterns || !t->tp[p].alloc) && !allocTreePart(t, p)) return false;
     }
     return true;
 }
@@ -1760,6 +1771,8 @@ int p4g_pokeSequences(p4g_part *p, const char *seqs)
         p->sequences[i] = c;
     }
     p->dev.valid = false;
+    p->dSeqValid = false;
+    p->simCats.clear();
     return 0;
 }
 
@@ -1819,6 +1832,7 @@ int p4g_setGlobalInvarSitesVec(p4g_part *p)
 
 int p4g_partNPatterns(const p4g_part *p) { return p ? p->nPatterns : -1; }
 int p4g_partNChar(const p4g_part *p) { return p ? p->nChar : -1; }
+int p4g_partNTax(const p4g_part *p) { return p ? p->nTax : -1; }
 int p4g_partDim(const p4g_part *p) { return p ? p->dim : -1; }
 
 int p4g_partGetPatterns(const p4g_part *p, int *patterns, int *counts, int *seqPosPatternIndex,
@@ -2218,7 +2232,7 @@ int p4g_setPrams(p4g_tree *t, int pNum)
 {
     clearError();
     if (!t) return 1;
-    if (!treeReady(t)) return 1;
+    if (!treeReady(t, false)) return 1;
     if (pNum >= t->nParts) {
         setError("setPrams: part out of range");
         return 1;
@@ -2763,30 +2777,29 @@ void p4g_philox4x64(const unsigned long long *counter, const unsigned long long 
 }
 
 // ------------------------------------------------------ ancestral draws
-int p4g_drawAncStates(p4g_tree *t, int pNum, int first, int n, const double *u, unsigned long long seed, int *draw)
+}  // extern "C"
+
+namespace p4g {
+// Launch the root draws of sites [first, first + n) of part pNum; the draws
+// land in the tree part's buffer (*dDraw, device, [n][4]), valid until the
+// next draw on that part.  u: host uniforms or null.
+static bool ancLaunch(Tree *t, int pNum, int first, int n, const double *u, uint64_t seed, int **dDrawOut)
 {
-    clearError();
-    if (!t || !draw || pNum < 0 || pNum >= t->nParts || first < 0 || n < 0) {
-        setError("drawAncStates: bad arguments");
-        return 1;
-    }
-    if (n == 0) return 0;
-    if (!treeReady(t)) return 1;
     Part *dp = t->data->parts[pNum];
     TreePart &tp = t->tp[pNum];
     ModelPart *mp = t->model->parts[pNum];
     if ((long long)first + n > dp->dev.nSites) {
         setError("drawAncStates: sites [" + std::to_string(first) + ", " + std::to_string(first + n) +
                  ") run past the part's " + std::to_string(dp->dev.nSites) + " sites");
-        return 1;
+        return false;
     }
     if (tp.K > ANC_MAX_K) {
         setError("drawAncStates: at most 16 gamma categories");
-        return 1;
+        return false;
     }
     // bring the partials up to date (Tree.ancestralStateDraw evaluates first)
     double v = 0.0;
-    if (!evalPart(t, pNum, !tp.everComputed, false, &v)) return 1;
+    if (!evalPart(t, pNum, !tp.everComputed, false, &v)) return false;
     std::vector<DerivSib> kids;
     for (int c = t->nodes[t->root]->leftChild; c >= 0; c = t->nodes[c]->sibling) {
         const Node *cn = t->nodes[c];
@@ -2798,12 +2811,12 @@ int p4g_drawAncStates(p4g_tree *t, int pNum, int first, int n, const double *u, 
     hipStream_t s = stream();
     if (need > tp.ancCap) {
         if (tp.ancBuf) {
-            if (!hipOk(hipStreamSynchronize(s), "sync")) return 1;
+            if (!hipOk(hipStreamSynchronize(s), "sync")) return false;
             (void)hipFree(tp.ancBuf);
             tp.ancBuf = nullptr;
             tp.ancCap = 0;
         }
-        if (!hipOk(hipMalloc(&tp.ancBuf, need), "hipMalloc ancestral draw buffer")) return 1;
+        if (!hipOk(hipMalloc(&tp.ancBuf, need), "hipMalloc ancestral draw buffer")) return false;
         tp.ancCap = need;
     }
     DerivSib *dKids = reinterpret_cast<DerivSib *>(tp.ancBuf);
@@ -2832,14 +2845,36 @@ int p4g_drawAncStates(p4g_tree *t, int pNum, int first, int n, const double *u, 
     a.first = first;
     a.n = n;
     a.u = dU;
-    a.seed = (uint64_t)seed;
+    a.seed = seed;
     a.draw = dDraw;
     if ((!kids.empty() &&
          !hipOk(hipMemcpyAsync(dKids, kids.data(), kids.size() * sizeof(DerivSib), hipMemcpyHostToDevice, s),
                 "stage root children")) ||
         (u && !hipOk(hipMemcpyAsync(dU, u, (size_t)n * 8, hipMemcpyHostToDevice, s), "stage uniforms")) ||
-        !hipOk(launchAncDraw(a, s), "launch ancestral draw kernel") ||
-        !hipOk(hipMemcpyAsync(draw, dDraw, (size_t)n * 16, hipMemcpyDeviceToHost, s), "copy draws") ||
+        !hipOk(launchAncDraw(a, s), "launch ancestral draw kernel")) {
+        (void)hipStreamSynchronize(s);
+        return false;
+    }
+    *dDrawOut = dDraw;
+    return true;
+}
+}  // namespace p4g
+
+extern "C" {
+
+int p4g_drawAncStates(p4g_tree *t, int pNum, int first, int n, const double *u, unsigned long long seed, int *draw)
+{
+    clearError();
+    if (!t || !draw || pNum < 0 || pNum >= t->nParts || first < 0 || n < 0) {
+        setError("drawAncStates: bad arguments");
+        return 1;
+    }
+    if (n == 0) return 0;
+    if (!treeReady(t)) return 1;
+    int *dDraw = nullptr;
+    hipStream_t s = stream();
+    if (!ancLaunch(t, pNum, first, n, u, (uint64_t)seed, &dDraw)) return 1;
+    if (!hipOk(hipMemcpyAsync(draw, dDraw, (size_t)n * 16, hipMemcpyDeviceToHost, s), "copy draws") ||
         !hipOk(hipStreamSynchronize(s), "ancestral draw kernel")) {
         (void)hipStreamSynchronize(s);
         return 1;
@@ -2851,6 +2886,282 @@ int p4g_drawAncStates(p4g_tree *t, int pNum, int first, int n, const double *u, 
             return 1;
         }
     return 0;
+}
+
+// ------------------------------------------------------------ simulation
+}  // extern "C"
+
+namespace p4g {
+// The part's device sequence buffer, [nTax][nChar] bytes rounded up to 4.
+static bool partSeqBuffer(Part *p)
+{
+    const size_t need = ((size_t)p->nTax * p->nChar + 3) & ~(size_t)3;
+    if (p->dSeq && p->dSeqCap >= need) return true;
+    if (p->dSeq) {
+        if (!hipOk(hipStreamSynchronize(stream()), "sync")) return false;
+        devFree(p->dSeq);
+    }
+    p->dSeqCap = 0;
+    p->dSeqValid = false;
+    if (!hipOk(hipMalloc(&p->dSeq, std::max<size_t>(need, 4)), "hipMalloc sequences")) return false;
+    p->dSeqCap = std::max<size_t>(need, 4);
+    return true;
+}
+}  // namespace p4g
+
+extern "C" {
+
+int p4g_simulate(p4g_tree *t, p4g_tree *refTree, unsigned long long seed)
+{
+    clearError();
+    if (!t) {
+        setError("simulate: bad arguments");
+        return 1;
+    }
+    if (refTree == t) {
+        setError("simulate: the tree and the reference tree are the same tree");
+        return 1;
+    }
+    if (!treeReady(t, false)) return 1;
+    if (refTree) {
+        if (!treeReady(refTree)) return 1;
+        if (refTree->nParts != t->nParts) {
+            setError("simulate: the reference tree has a different number of parts");
+            return 1;
+        }
+        for (int p = 0; p < t->nParts; p++)
+            if (refTree->data->parts[p]->dev.nSites != t->data->parts[p]->nChar) {
+                setError("simulate: part " + std::to_string(p) + " of the reference tree has " +
+                         std::to_string(refTree->data->parts[p]->dev.nSites) + " sites, the tree's " +
+                         std::to_string(t->data->parts[p]->nChar));
+                return 1;
+            }
+    }
+    hipStream_t s = stream();
+    const int N = t->nNodes;
+    // preorder non-root nodes, parents, tip rows
+    std::vector<int> host;
+    host.reserve(3 * N);
+    for (int j = 0; j < N; j++) {
+        const int n = t->preOrder[j];
+        if (n == NO_ORDER || n == t->root) continue;
+        if (n < 0 || n >= N) {
+            setError("preOrder holds an out-of-range node number");
+            return 1;
+        }
+        host.push_back(n);
+    }
+    const int nPre = (int)host.size();
+    host.resize((size_t)nPre + 2 * N);
+    for (int n = 0; n < N; n++) {
+        host[nPre + n] = t->nodes[n]->parent;
+        host[nPre + N + n] = t->nodes[n]->isLeaf ? t->nodes[n]->seqNum : -1;
+    }
+    for (int p = 0; p < t->nParts; p++) {
+        Part *dp = t->data->parts[p];
+        ModelPart *mp = t->model->parts[p];
+        const int nChar = dp->nChar, S = mp->dim, nTax = dp->nTax;
+        for (int n = 0; n < N; n++)
+            if (t->nodes[n]->isLeaf && (t->nodes[n]->seqNum < 0 || t->nodes[n]->seqNum >= nTax)) {
+                setError("simulate: leaf " + std::to_string(n) + " has no sequence row");
+                return 1;
+            }
+        const int rc = t->nodes[t->root]->compNums[p];
+        if (rc < 0 || rc >= mp->nComps || !mp->comps[rc].val) {
+            setError("simulate: root comp out of range");
+            return 1;
+        }
+        // P of every non-root node from the current model and branch lengths
+        // (Pf/p4_treeSim.c:362-368)
+        for (int n = 0; n < N; n++)
+            if (n != t->root && !bigPDecksPart(t, t->nodes[n], p)) return 1;
+        if (!flushParams(t, p)) {
+            (void)hipStreamSynchronize(s);
+            return 1;
+        }
+        int *dRef = nullptr;
+        if (refTree && !ancLaunch(refTree, p, 0, nChar, nullptr, (uint64_t)seed ^ 0x9E3779B97F4A7C15ull, &dRef))
+            return 1;
+        auto up256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        const size_t offComp = up256(host.size() * 4), offStates = offComp + up256((size_t)S * 8),
+                     offCat = offStates + up256((size_t)N * nChar), need = offCat + (size_t)nChar * 4;
+        if (need > t->simCap) {
+            if (!hipOk(hipStreamSynchronize(s), "sync")) return 1;
+            if (t->simBuf) (void)hipFree(t->simBuf);
+            t->simBuf = nullptr;
+            t->simCap = 0;
+            if (!hipOk(hipMalloc(&t->simBuf, need), "hipMalloc simulation buffer")) return 1;
+            t->simCap = need;
+        }
+        if (!partSeqBuffer(dp)) return 1;
+        SimArgs a{};
+        a.dim = S;
+        a.K = mp->nCat;
+        a.nChar = nChar;
+        a.nNodes = N;
+        a.pNum = p;
+        a.root = t->root;
+        a.nPre = nPre;
+        a.pre = reinterpret_cast<const int *>(t->simBuf);
+        a.parent = a.pre + nPre;
+        a.seqNum = a.pre + nPre + N;
+        a.PT64 = t->tp[p].PT64;
+        a.rootComp = reinterpret_cast<const double *>(t->simBuf + offComp);
+        a.pInvar = mp->pInvar;
+        a.refDraw = dRef;
+        a.seed = (uint64_t)seed;
+        a.states = reinterpret_cast<uint8_t *>(t->simBuf + offStates);
+        a.seq = dp->dSeq;
+        a.simCat = reinterpret_cast<int *>(t->simBuf + offCat);
+        std::vector<int8_t> seq((size_t)nTax * nChar, 0);
+        dp->simCats.assign(nChar, 0);
+        if (!hipOk(hipMemcpyAsync(t->simBuf, host.data(), host.size() * 4, hipMemcpyHostToDevice, s), "stage nodes") ||
+            !hipOk(hipMemcpyAsync(t->simBuf + offComp, mp->comps[rc].val, (size_t)S * 8, hipMemcpyHostToDevice, s),
+                   "stage root comp") ||
+            !hipOk(launchFillU32(reinterpret_cast<unsigned *>(dp->dSeq), 0u, (int)(dp->dSeqCap / 4), s),
+                   "clear sequences") ||
+            !hipOk(launchSimulate(a, s), "launch simulation kernel") ||
+            !hipOk(hipMemcpyAsync(seq.data(), dp->dSeq, seq.size(), hipMemcpyDeviceToHost, s), "copy sequences") ||
+            !hipOk(hipMemcpyAsync(dp->simCats.data(), a.simCat, (size_t)nChar * 4, hipMemcpyDeviceToHost, s),
+                   "copy categories") ||
+            !hipOk(hipStreamSynchronize(s), "simulation kernel")) {
+            (void)hipStreamSynchronize(s);
+            return 1;
+        }
+        // the part now holds the simulated sequences; its patterns are
+        // stale until they are made again (Pf/p4_treeSim.c:64-69)
+        dp->sequences.assign(seq.begin(), seq.end());
+        dp->dSeqValid = true;
+        dp->nPatterns = 0;
+        dp->patterns.clear();
+        dp->patternCounts.clear();
+        dp->invarSet = false;
+        dp->siteLikes.clear();
+        dp->dev.valid = false;
+    }
+    return 0;
+}
+
+// Pf/part.c:604-680
+int p4g_symbolSequences(const p4g_part *p, char *out)
+{
+    clearError();
+    if (!p || !out) return 1;
+    const size_t n = (size_t)p->nTax * p->nChar;
+    if (p->nChar < 1 || p->sequences.size() != n) {
+        setError("symbolSequences: no sequences");
+        return 1;
+    }
+    for (size_t i = 0; i < n; i++) {
+        const int c = p->sequences[i];
+        if (c >= 0 && c < (int)p->symbols.size())
+            out[i] = p->symbols[c];
+        else if (c == GAP_CODE)
+            out[i] = '-';
+        else if (c == QMARK_CODE)
+            out[i] = '?';
+        else if (c >= EQUATES_BASE && c < EQUATES_BASE + p->nEquates)
+            out[i] = p->equateSymbols[c - EQUATES_BASE];
+        else {
+            setError("symbolSequences: character number " + std::to_string(c) + " is not recognized");
+            return 1;
+        }
+    }
+    return 0;
+}
+
+int p4g_partSimCats(const p4g_part *p, int *cats)
+{
+    clearError();
+    if (!p || !cats) return 1;
+    if ((int)p->simCats.size() != p->nChar) {
+        setError("partSimCats: the part holds no simulation");
+        return 1;
+    }
+    std::copy(p->simCats.begin(), p->simCats.end(), cats);
+    return 0;
+}
+
+int p4g_partSeqStats(p4g_part *p, int *compCounts, int *nConstant, double *meanNChars, int *nNonState)
+{
+    clearError();
+    if (!p || !compCounts || !nConstant || !meanNChars || !nNonState) {
+        setError("partSeqStats: bad arguments");
+        return 1;
+    }
+    if (!deviceReady()) return 1;
+    hipStream_t s = stream();
+    const size_t n = (size_t)p->nTax * p->nChar;
+    if (n == 0 || p->sequences.size() != n) {
+        setError("partSeqStats: the part has no sequences");
+        return 1;
+    }
+    if (!p->dSeqValid) {
+        std::vector<int8_t> h(n);
+        for (size_t i = 0; i < n; i++) {
+            const int c = p->sequences[i];
+            h[i] = (int8_t)(c >= 0 && c < p->dim ? c : -1);
+        }
+        if (!partSeqBuffer(p)) return 1;
+        if (!hipOk(hipMemcpyAsync(p->dSeq, h.data(), n, hipMemcpyHostToDevice, s), "copy sequences") ||
+            !hipOk(hipStreamSynchronize(s), "copy sequences"))
+            return 1;
+        p->dSeqValid = true;
+    }
+    const size_t nCounts = (size_t)p->nTax * p->dim;
+    int *d = nullptr;
+    if (!hipOk(hipMalloc(&d, (nCounts + 4) * 4), "hipMalloc statistics")) return 1;
+    std::vector<int> h(nCounts + 4);
+    const bool ok = hipOk(launchFillU32(reinterpret_cast<unsigned *>(d), 0u, (int)(nCounts + 4), s), "clear") &&
+                    hipOk(launchSeqStats(p->dSeq, p->nTax, p->nChar, p->dim, d + 4, d, s), "launch statistics") &&
+                    hipOk(hipMemcpyAsync(h.data(), d, h.size() * 4, hipMemcpyDeviceToHost, s), "copy statistics") &&
+                    hipOk(hipStreamSynchronize(s), "statistics kernel");
+    (void)hipFree(d);
+    if (!ok) return 1;
+    std::copy(h.begin() + 4, h.end(), compCounts);
+    *nConstant = h[0];
+    *meanNChars = (double)h[1] / (double)p->nChar;
+    *nNonState = h[2];
+    return 0;
+}
+
+double p4g_partBigXSquared(p4g_part *p)
+{
+    clearError();
+    if (!p) return -3.0;
+    std::vector<int> c((size_t)p->nTax * p->dim);
+    int nConst = 0, nOther = 0;
+    double mean = 0.0;
+    if (p4g_partSeqStats(p, c.data(), &nConst, &mean, &nOther)) return -3.0;
+    if (nOther) return -2.0;   // Pf/part.c:1519-1525: no gaps or ambiguities
+    // Pf/part.c:1530-1560, in its order of operations
+    const double oneOverNTax = 1.0 / p->nTax;
+    double xSq = 0.0;
+    std::vector<double> ex(p->dim);
+    for (int j = 0; j < p->dim; j++) {
+        double soc = 0.0;
+        for (int t = 0; t < p->nTax; t++) soc += c[(size_t)t * p->dim + j];
+        ex[j] = oneOverNTax * soc;
+    }
+    for (int t = 0; t < p->nTax; t++)
+        for (int j = 0; j < p->dim; j++)
+            if (ex[j] != 0.0) {
+                const double d = c[(size_t)t * p->dim + j] - ex[j];
+                xSq += (d * d) / ex[j];
+            }
+    return xSq;
+}
+
+double p4g_getUnconstrainedLogLike(const p4g_part *p)
+{
+    clearError();
+    if (!p || p->nPatterns < 1) {
+        setError("getUnconstrainedLogLike: no patterns");
+        return 0.0;
+    }
+    double dsum = 0.0;   // Pf/part.c:682-714
+    for (int i = 0; i < p->nPatterns; i++) dsum += p->patternCounts[i] * std::log((double)p->patternCounts[i]);
+    return dsum - p->nChar * std::log((double)p->nChar);
 }
 
 // -------------------------------------------------------------- batches

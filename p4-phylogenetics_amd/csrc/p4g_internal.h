@@ -78,6 +78,12 @@ struct Part {
     // CPO: per pattern, the sum over samples of 1/like, in GPU memory
     double *cpoSum = nullptr;
     int cpoPatterns = 0, cpoSamples = 0;
+    // device copy of `sequences` ([nTax][nChar] codes), written by the
+    // simulation kernel or uploaded for the sequence statistics
+    int8_t *dSeq = nullptr;
+    size_t dSeqCap = 0;             // bytes, a multiple of 4
+    bool dSeqValid = false;
+    std::vector<int> simCats;       // [nChar] after a simulation: category, -1 invariant
     ~Part();
 };
 
@@ -301,6 +307,8 @@ struct Tree {
     int precision = FP64;
     std::vector<TreePart> tp;
     std::vector<int> slotOf;  // node -> cl slot (-1 none)
+    char *simBuf = nullptr;   // simulation: node lists, root comp, states [nNodes][nChar], categories
+    size_t simCap = 0;
     ~Tree();
 };
 
@@ -528,6 +536,28 @@ struct AncArgs {
     int *draw;                   // [n][4] chStNum, catNum, isInvar, invarChNum
 };
 hipError_t launchAncDraw(const AncArgs &a, hipStream_t s);
+
+// ------------------------------------------------ simulation (p4g_sim.hip)
+struct SimArgs {
+    int dim, K, nChar, nNodes, pNum, root;
+    int nPre;
+    const int *pre;              // [nPre] non-root nodes in preorder
+    const int *parent;           // [nNodes]
+    const int *seqNum;           // [nNodes] tip row of a leaf, else -1
+    const double *PT64;          // P^T of the tree part
+    const double *rootComp;      // [dim]
+    double pInvar;
+    const int *refDraw;          // [nChar][4] root draws of a reference tree, or null
+    uint64_t seed;
+    uint8_t *states;             // [nNodes][nChar]
+    int8_t *seq;                 // [nTax][nChar] the leaves' states
+    int *simCat;                 // [nChar] category, -1 for an invariant site
+};
+hipError_t launchSimulate(const SimArgs &a, hipStream_t s);
+// Over the sites of seq [nTax][nChar]: per-taxon counts of each state
+// (counts [nTax][dim], added to), and acc[0] += constant sites, acc[1] +=
+// distinct states summed over sites, acc[2] += codes outside 0..dim-1.
+hipError_t launchSeqStats(const int8_t *seq, int nTax, int nChar, int dim, int *counts, int *acc, hipStream_t s);
 // sum[p] += 1 / like[p] for p < nPat (the CPO sums)
 hipError_t launchCpoAccum(const double *like, double *sum, int nPat, hipStream_t s);
 

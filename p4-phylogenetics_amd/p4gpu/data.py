@@ -168,6 +168,42 @@ class Data(object):
     def nParts(self):
         return len(self.parts)
 
+    def dupe(self):
+        """A copy with new C parts (p4/data.py:670-683)."""
+        if self.alignments:
+            als = [Alignment(a.taxNames, list(a.sequences), a.dataType, a.symbols, dict(a.equates))
+                   for a in self.alignments]
+            return Data(alignments=als)
+        parts = []
+        for p in self.parts:
+            pats, cnts, _, _, _ = p.patterns()
+            parts.append(Part.fromPatterns(pats, cnts, p.dataType, p.symbols, p.equates))
+        return Data(parts=parts, taxNames=self.taxNames)
+
+    def resetSequencesFromParts(self):
+        """The alignments' sequences from their parts' C sequences (after a
+        simulation; p4/data.py:305-310, p4/alignment.py)."""
+        for a, p in zip(self.alignments, self.parts):
+            s = pf.symbolSequences(p.cPart)
+            n = pf.lib().p4g_partNChar(p.cPart)
+            a.sequences = [s[i * n:(i + 1) * n] for i in range(p.nTax)]
+            a.length = n
+
+    def simpleBigXSquared(self):
+        """Per part, the compositional chi-square over taxa, no gaps or
+        ambiguities allowed (p4/data.py:636-651; -2.0 if there are)."""
+        return [pf.partBigXSquared(p.cPart) for p in self.parts]
+
+    def simpleConstantSitesCount(self):
+        """Per part, sites whose non-gap characters are all one state
+        (p4/data.py:653-668)."""
+        return [pf.partSimpleConstantSitesCount(p.cPart) for p in self.parts]
+
+    def meanNCharsPerSite(self):
+        """Per part, the mean number of different states per site
+        (p4/data.py:752-767)."""
+        return [pf.partMeanNCharsPerSite(p.cPart) for p in self.parts]
+
     def _setCStuff(self):
         """p4/data.py:261-278."""
         self.cData = pf.newData(self.nTax, self.nParts)

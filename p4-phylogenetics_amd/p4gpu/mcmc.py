@@ -372,7 +372,7 @@ class Mcmc(object):
     gives T = 0, 0.1, 0.2, ... as with p4's chainTempDiffs, untuned."""
 
     def __init__(self, aTree, nChains=4, runNum=0, sampleInterval=100, chainTemps=None,
-                 seed=None, verbose=True, writeFiles=False, proposalWeights=None):
+                 seed=None, verbose=True, writeFiles=False, proposalWeights=None, simulate=None):
         if aTree.data is None or aTree.model is None:
             raise P4Error("Mcmc: the tree needs data and a model")
         self.rng = random.Random(seed)
@@ -435,6 +435,21 @@ class Mcmc(object):
         self.cpo_startGen = 0
         for dp in aTree.data.parts:
             pf.cpoReset(dp.cPart)
+        # Posterior-predictive simulation at every sample (p4/mcmc.py:936,
+        # 2654-2657, 3461-3514): a bit mask of the statistics recorded for a
+        # data set simulated on the cold chain's tree and model -- 1 the
+        # unconstrained log likelihood, 2 the compositional chi-square,
+        # 4 the mean number of states per site, 16 the constant sites, each
+        # per part, into samples['sim'] rows [gen + 1, ...].
+        self.simulate = int(simulate or 0)
+        if self.simulate & 8:
+            raise P4Error("Mcmc: simulate bit 8 (compStatFromCharFreqs) is not provided")
+        self.simTree = None
+        self.samples['sim'] = []
+        if self.simulate:
+            self.simTree = aTree.dupe()
+            self.simTree._data = aTree.data.dupe()
+            self.simTree.calcLogLike(verbose=0)
 
     def addLearnedProposal(self, learned, weight):
         """Add the learned joint branch-length move (p4gpu.learned)."""
@@ -530,8 +545,28 @@ class Mcmc(object):
         self.samples['prams'].append(pr)
         if self.doCpo and self.gen + 1 > self.cpo_startGen:
             pf.cpoAccumulate(t.cTree)
+        if self.simulate:
+            self._doSimulate(t)
         for f in self.sampleHooks:
             f(t)
+
+    def _doSimulate(self, curTree):
+        """p4/mcmc.py:3461-3514: simulate on the cold chain's tree and
+        model, then the statistics of the simulate bit mask."""
+        curTree.copyToTree(self.simTree)
+        curTree.model.copyValsTo(self.simTree.model)
+        self.simTree.simulate(resetSequences=False, seed=self.rng.getrandbits(64))
+        d = self.simTree.data
+        row = [self.gen + 1]
+        if 1 & self.simulate:
+            row += [pf.getUnconstrainedLogLike(p.cPart) for p in d.parts]
+        if 2 & self.simulate:
+            row += d.simpleBigXSquared()
+        if 4 & self.simulate:
+            row += d.meanNCharsPerSite()
+        if 16 & self.simulate:
+            row += d.simpleConstantSitesCount()
+        self.samples['sim'].append(row)
 
     # ------------------------------------------------------------- CPO
     @property

@@ -90,6 +90,7 @@ def _declare():
     _sig("p4g_setGlobalInvarSitesVec", I, V)
     _sig("p4g_partNPatterns", I, V)
     _sig("p4g_partNChar", I, V)
+    _sig("p4g_partNTax", I, V)
     _sig("p4g_partDim", I, V)
     _sig("p4g_partGetPatterns", I, V, V, V, V, V, V)
     _sig("p4g_partSetPatterns", I, V, I, V, V)
@@ -146,6 +147,12 @@ def _declare():
     _sig("p4g_partNSites", I, V)
     _sig("p4g_drawAncStates", I, V, I, I, I, V, ctypes.c_ulonglong, V)
     _sig("p4g_philox4x64", None, V, V, V)
+    _sig("p4g_simulate", I, V, V, ctypes.c_ulonglong)
+    _sig("p4g_symbolSequences", I, V, c_char_p)
+    _sig("p4g_partSimCats", I, V, V)
+    _sig("p4g_partSeqStats", I, V, V, V, V, V)
+    _sig("p4g_partBigXSquared", D, V)
+    _sig("p4g_getUnconstrainedLogLike", D, V)
     _sig("p4g_newBatch", V, I, V)
     _sig("p4g_mlpNew", V, I, I, I, ctypes.c_ulonglong)
     _sig("p4g_mlpFree", None, V)
@@ -576,6 +583,64 @@ def p4_drawAncState(cTree, partNum, seqPos, draw):
     import random
     d = drawAncStates(cTree, partNum, seqPos, 1, seed=random.getrandbits(64))
     draw[:4] = d[0]
+
+
+# --------------------------------------------------------------- simulation
+def p4_simulate(cTree, refCTree, seed):
+    """pf.p4_simulate (Pf/p4_treeSim.c:14-590) on the GPU: new sequences
+    for every part of the tree's data; seed keys the Philox stream."""
+    _ok(lib().p4g_simulate(cTree, refCTree, ctypes.c_ulonglong(seed & 0xFFFFFFFFFFFFFFFF)), "p4_simulate")
+
+
+def partNTax(cPart):
+    return lib().p4g_partNTax(cPart)
+
+
+def symbolSequences(cPart):
+    L = lib()
+    n = L.p4g_partNChar(cPart) * partNTax(cPart)
+    buf = ctypes.create_string_buffer(n + 1)
+    _ok(L.p4g_symbolSequences(cPart, buf), "symbolSequences")
+    return buf.raw[:n].decode()
+
+
+def partSimCats(cPart):
+    """Per site of the last simulation: gamma category, -1 for invariant."""
+    out = numpy.zeros(lib().p4g_partNChar(cPart), numpy.int32)
+    _ok(lib().p4g_partSimCats(cPart, out.ctypes.data), "partSimCats")
+    return out
+
+
+def partSeqStats(cPart):
+    """(compCounts [nTax][dim], nConstant, meanNChars, nNonState)."""
+    L = lib()
+    counts = numpy.zeros((partNTax(cPart), L.p4g_partDim(cPart)), numpy.int32)
+    nc, other = ctypes.c_int(), ctypes.c_int()
+    mean = ctypes.c_double()
+    _ok(L.p4g_partSeqStats(cPart, counts.ctypes.data, ctypes.byref(nc), ctypes.byref(mean), ctypes.byref(other)),
+        "partSeqStats")
+    return counts, nc.value, mean.value, other.value
+
+
+def partBigXSquared(cPart):
+    v = lib().p4g_partBigXSquared(cPart)
+    if v == -3.0:
+        _err("partBigXSquared")
+    return v
+
+
+def partSimpleConstantSitesCount(cPart):
+    return partSeqStats(cPart)[1]
+
+
+def partMeanNCharsPerSite(cPart):
+    return partSeqStats(cPart)[2]
+
+
+def getUnconstrainedLogLike(cPart):
+    if lib().p4g_partNPatterns(cPart) < 1:
+        _err("getUnconstrainedLogLike")
+    return lib().p4g_getUnconstrainedLogLike(cPart)
 
 
 def p4_getCondLikes(cNode, pNum, nCat, dim, nPatterns):

@@ -555,6 +555,35 @@ class Tree(object):
             out += pf.getSiteLikes(p.cPart)
         return out
 
+    def simulate(self, calculatePatterns=True, resetSequences=True, refTree=None, seed=None):
+        """Simulate into the attached data (p4/tree.py:9527-9627), every part
+        on the GPU, one thread per site.  With refTree (a tree, model and
+        other data of the same shape), each site's root state, category and
+        invariance are drawn from refTree's root conditional likelihoods, as
+        the reference's PhyloBayes-style option does.  seed keys the
+        counter-based generator (default: from Python's random module).
+        calculatePatterns makes the parts' patterns again; resetSequences
+        brings the new sequences into the alignments."""
+        if refTree is not None:
+            if refTree.data is None or refTree.model is None:
+                raise P4Error("Tree.simulate(): refTree needs data and a model")
+            if refTree.data is self.data:
+                raise P4Error("Tree.simulate(): refTree must have its own data")
+            if refTree.data.nParts != self.data.nParts or refTree.data.nTax != self.data.nTax:
+                raise P4Error("Tree.simulate(): refTree's data differs in shape")
+            refTree.calcLogLike(verbose=0)
+        self._commonCStuff()
+        if seed is None:
+            seed = random.getrandbits(64)
+        pf.p4_simulate(self.cTree, refTree.cTree if refTree is not None else None, int(seed))
+        if calculatePatterns:
+            for p in self.data.parts:
+                pf.makePatterns(p.cPart)
+                pf.setGlobalInvarSitesVec(p.cPart)
+                p.nPatterns = pf.partPatternCount(p.cPart)
+        if resetSequences and self.data.alignments:
+            self.data.resetSequencesFromParts()
+
     def ancestralStateDraw(self, seed=None):
         """One draw of the root's character states, all sites, as a string
         (p4/tree.py:9640-9676): for each site a (gamma category, state) or
