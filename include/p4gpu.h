@@ -242,6 +242,24 @@ int p4g_getCondLikes2(const p4g_node *n, int pNum, double *out);
 int p4g_getBigPDeck1stD(const p4g_node *n, int pNum, double *out);
 int p4g_getBigPDeck2ndD(const p4g_node *n, int pNum, double *out);
 
+/* ---- MCMC state sync without whole-tree copies (no single counterpart:
+ * replaces the per-node marking loops of p4/chain.py's proposals, through
+ * p4_setConditionalLikelihoodsOfInternalNodePart, and the doAll copies of
+ * Pf/p4_treeCopyVerify.c:7-61 in Chain's accept/reject step,
+ * p4/chain.py:1351-1531).
+ * p4g_markPath: node nodeNum and its ancestors (internal nodes and a leaf
+ * root) marked for recomputation in part pNum, or every part (-1).
+ * p4g_markAll: every internal node (and a leaf root) of part pNum or all. */
+int p4g_markPath(p4g_tree *t, int nodeNum, int pNum);
+int p4g_markAll(p4g_tree *t, int pNum);
+/* dst takes src's partial slots, scale exponents and P matrices wherever
+ * either tree changed them since the two were last synced (slots written
+ * by an evaluation, P blocks of marked nodes; everything after buffers were
+ * made or re-laid out), for every part in one copy launch; dirty marks,
+ * part likelihoods and lnL follow.  Model values and eigensystems are
+ * p4g_copyModelPrams' job.  The copy is queued on the library's stream. */
+int p4g_syncChanged(p4g_tree *src, p4g_tree *dst);
+
 /* ---- CPO / LPML (the doCpo step of Mcmc's sampling, p4/mcmc.py:3286-3297,
  * and Mcmc.lpml, p4/mcmc.py:3730-3748).  p4g_cpoAccumulate evaluates tree t
  * fully, as p4_treeLogLike(t, 1) (Pf/p4_tree.c:868-1021) does, keeps each

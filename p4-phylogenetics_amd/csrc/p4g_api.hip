@@ -203,6 +203,9 @@ Tree::~Tree()
     for (auto &t : tp) freeTreePart(t);
     if (simBuf) (void)hipFree(simBuf);
     simBuf = nullptr;
+    hostFree(hRanges);
+    if (syncEv) (void)hipEventDestroy(syncEv);
+    syncEv = nullptr;
 }
 
 // ------------------------------------------------------------- data side
@@ -418,6 +421,9 @@ static bool allocTreePart(Tree *t, int pNum)
     for (int n = 0; n < t->nNodes; n++)
         if (t->nodes[n] && !t->nodes[n]->isLeaf) t->slotOf[n] = ns++;
     tp.nSlots = ns + 1;
+    tp.chSlot.assign(tp.nSlots, 1);
+    tp.chP.assign(t->nNodes, 1);
+    tp.chAll = true;
     tp.nEig = mp->nComps * mp->nRMatrices;
     const size_t nPS = (size_t)t->nNodes * K * S * S;
     // blocks of the launch: the largest count any pruning kernel may use
@@ -481,9 +487,9 @@ static bool allocTreePart(Tree *t, int pNum)
     return true;
 }
 
-// needPatterns = false: a part whose tree state exists is left as it is
-// (parameters and simulation do not read the patterns, which a simulation
-// has cleared).
+// needPatterns = false: a part whose patterns a simulation has cleared
+// keeps the tree state it has (parameters and simulation do not read the
+// patterns).
 static bool treeReady(Tree *t, bool needPatterns = true)
 {
     if (!deviceReady()) return false;
@@ -509,7 +515,8 @@ static bool treeReady(Tree *t, bool needPatterns = true)
             setError("model or data part missing");
             return false;
         }
-        if ((needPatterns || !t->tp[p].alloc) && !allocTreePart(t, p)) return false;
+        if ((needPatterns || !t->tp[p].alloc || t->data->parts[p]->nPatterns > 0) && !allocTreePart(t, p))
+            return false;
     }
     return true;
 }
@@ -564,6 +571,7 @@ static void markP(Tree *t, int pNum, const Node *nd)
     tp.hEs[n] = nd->compNums[pNum] * mp->nRMatrices + nd->rMatrixNums[pNum];
     tp.pPending[n] = 1;
     tp.pValid[n] = 1;
+    if ((size_t)n < tp.chP.size()) tp.chP[n] = 1;
 }
 
 // Pack the pending eigensystem slots and P items of one part into the
@@ -893,6 +901,9 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
     std::vector<OpRec> ops;
     std::vector<KidRec> kids;
     if (!collectOps(t, pNum, full, ops, kids)) return false;
+    tp.progSlots.clear();
+    for (const OpRec &op : ops)
+        if (op.dst >= 0) tp.progSlots.push_back(op.dst);
     bool ok;
     if (tp.wave) {
         *nOpsOut = (int)ops.size();
@@ -1361,6 +1372,8 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps, std::vector<Defer
     const Node *root = t->nodes[t->root];
     int nKids = 0;
     if (!buildProgram(t, pNum, full, nOps, &nKids)) return false;
+    for (int sl : tp.progSlots)
+        if ((size_t)sl < tp.chSlot.size()) tp.chSlot[sl] = 1;
     const int rc = root->compNums[pNum];
     if (rc < 0 || rc >= mp->nComps || !mp->comps[rc].val) {
         setError("root comp out of range");
@@ -2379,6 +2392,51 @@ int p4g_setConditionalLikelihoodsOfInternalNode(p4g_node *n)
     return 0;
 }
 
+// Node n and its ancestors (internal nodes, and a leaf root) marked for
+// recomputation in part pNum or every part (-1): the walk Chain's proposals
+// make after a change at n, as one call.
+int p4g_markPath(p4g_tree *t, int nodeNum, int pNum)
+{
+    clearError();
+    if (!t || nodeNum < 0 || nodeNum >= t->nNodes || pNum < -1 || pNum >= t->nParts) {
+        setError("markPath: bad arguments");
+        return 1;
+    }
+    int guard = 0;
+    for (int n = nodeNum; n >= 0; n = t->nodes[n]->parent) {
+        if (!t->nodes[n] || ++guard > t->nNodes) {
+            setError("markPath: the parent chain does not end at the root");
+            return 1;
+        }
+        if (t->nodes[n]->isLeaf && n != t->root) continue;
+        for (int p = 0; p < t->nParts; p++) {
+            if (pNum >= 0 && p != pNum) continue;
+            TreePart &tp = t->tp[p];
+            if (tp.dirty.size() != (size_t)t->nNodes) tp.dirty.assign(t->nNodes, 1);
+            tp.dirty[n] = 1;
+        }
+    }
+    return 0;
+}
+
+// Every internal node (and a leaf root) of part pNum, or every part (-1).
+int p4g_markAll(p4g_tree *t, int pNum)
+{
+    clearError();
+    if (!t || pNum < -1 || pNum >= t->nParts) {
+        setError("markAll: bad arguments");
+        return 1;
+    }
+    for (int p = 0; p < t->nParts; p++) {
+        if (pNum >= 0 && p != pNum) continue;
+        TreePart &tp = t->tp[p];
+        if (tp.dirty.size() != (size_t)t->nNodes) tp.dirty.assign(t->nNodes, 1);
+        for (int n = 0; n < t->nNodes; n++)
+            if (t->nodes[n] && (!t->nodes[n]->isLeaf || n == t->root)) tp.dirty[n] = 1;
+    }
+    return 0;
+}
+
 // ---------------------------------------------------------- state sync
 static bool sameShape(Tree *a, Tree *b)
 {
@@ -2452,6 +2510,87 @@ int p4g_copyBigPDecks(p4g_tree *a, p4g_tree *b, int doAll)
             return 1;
         B.pValid = A.pValid;
     }
+    return 0;
+}
+
+// The MCMC state sync without whole-tree copies: dst takes src's partial
+// slots, scale exponents and P blocks wherever either tree changed them
+// since their last sync (the recomputed path of a proposal; everything after
+// a re-layout), all parts in one copy launch.  Replaces p4_copyCondLikes and
+// p4_copyBigPDecks with doAll (Pf/p4_treeCopyVerify.c:7-61) in Chain's
+// accept/reject step (p4/chain.py:1351-1531).
+int p4g_syncChanged(p4g_tree *a, p4g_tree *b)
+{
+    clearError();
+    if (!a || !b || a == b || !sameShape(a, b) || !treeReady(a, false) || !treeReady(b, false)) return 1;
+    if (a->precision != b->precision) {
+        setError("syncChanged: trees have different precision");
+        return 1;
+    }
+    hipStream_t s = stream();
+    // P matrices still pending in src are computed first
+    for (int p = 0; p < a->nParts; p++)
+        if (!flushParams(a, p)) {
+            (void)hipStreamSynchronize(s);
+            return 1;
+        }
+    std::vector<CopyRange> r;
+    for (int p = 0; p < a->nParts; p++) {
+        TreePart &A = a->tp[p], &B = b->tp[p];
+        if (A.nSlots != B.nSlots || A.clBytes != B.clBytes || A.wave != B.wave || A.sclPerSlot != B.sclPerSlot ||
+            A.K != B.K || A.dim != B.dim) {
+            setError("syncChanged: partial buffers differ");
+            return 1;
+        }
+        const bool all = A.chAll || B.chAll;
+        for (int sl = 0; sl < A.nSlots; sl++) {
+            if (!all && !A.chSlot[sl] && !B.chSlot[sl]) continue;
+            r.push_back({(const char *)A.cl + A.clBytes * sl, (char *)B.cl + A.clBytes * sl, A.clBytes});
+            if (A.scl)
+                r.push_back({A.scl + (size_t)sl * A.sclPerSlot, B.scl + (size_t)sl * A.sclPerSlot,
+                             sizeof(int) * A.sclPerSlot});
+        }
+        const size_t pe = (size_t)A.K * A.dim * A.dim;
+        for (int n = 0; n < a->nNodes; n++) {
+            if (!all && !A.chP[n] && !B.chP[n]) continue;
+            r.push_back({A.PT64 + (size_t)n * pe, B.PT64 + (size_t)n * pe, pe * 8});
+            r.push_back({A.PT32 + (size_t)n * pe, B.PT32 + (size_t)n * pe, pe * 4});
+        }
+    }
+    if (!r.empty()) {
+        // the descriptors live in pinned memory that the previous copy
+        // launch of this tree may still be reading
+        if (b->syncEv && !hipOk(hipEventSynchronize(b->syncEv), "sync")) return 1;
+        if ((int)r.size() > b->rangeCap) {
+            hostFree(b->hRanges);
+            b->rangeCap = 0;
+            const int cap = std::max<int>((int)r.size(), 64);
+            if (!hostAlloc(&b->hRanges, (size_t)cap, "hipHostMalloc copy ranges") ||
+                !hostDevPtr(b->hRanges, &b->dRanges, "copy ranges device address"))
+                return 1;
+            b->rangeCap = cap;
+        }
+        std::copy(r.begin(), r.end(), b->hRanges);
+        if (!b->syncEv && !hipOk(hipEventCreateWithFlags(&b->syncEv, hipEventDisableTiming), "event")) return 1;
+        if (!hipOk(launchCopyRanges(b->dRanges, (int)r.size(), s), "launch copy kernel") ||
+            !hipOk(hipEventRecord(b->syncEv, s), "record copy event")) {
+            (void)hipStreamSynchronize(s);
+            return 1;
+        }
+    }
+    for (int p = 0; p < a->nParts; p++) {
+        TreePart &A = a->tp[p], &B = b->tp[p];
+        B.dirty = A.dirty;
+        B.pValid = A.pValid;
+        B.everComputed = A.everComputed;
+        std::fill(A.chSlot.begin(), A.chSlot.end(), 0);
+        std::fill(B.chSlot.begin(), B.chSlot.end(), 0);
+        std::fill(A.chP.begin(), A.chP.end(), 0);
+        std::fill(B.chP.begin(), B.chP.end(), 0);
+        A.chAll = B.chAll = false;
+        if (a->partLikes && b->partLikes) b->partLikes[p] = a->partLikes[p];
+    }
+    b->logLike = a->logLike;
     return 0;
 }
 

@@ -292,6 +292,12 @@ struct TreePart {
     // ancestral draws: the root's children, uniforms and draws
     char *ancBuf = nullptr;
     size_t ancCap = 0;
+    // Changed since the last p4g_syncChanged with its partner tree: partial
+    // slots written and nodes whose P was marked (chAll: everything, after
+    // the buffers were made or re-laid out).
+    std::vector<char> chSlot, chP;
+    bool chAll = true;
+    std::vector<int> progSlots;    // slots the staged program writes
 };
 
 struct Tree {
@@ -309,6 +315,11 @@ struct Tree {
     std::vector<int> slotOf;  // node -> cl slot (-1 none)
     char *simBuf = nullptr;   // simulation: node lists, root comp, states [nNodes][nChar], categories
     size_t simCap = 0;
+    // p4g_syncChanged: copy descriptors in pinned memory (read by the copy
+    // kernel through their device address) and the event of the last copy
+    struct CopyRange *hRanges = nullptr, *dRanges = nullptr;
+    int rangeCap = 0;
+    hipEvent_t syncEv = nullptr;
     ~Tree();
 };
 
@@ -473,6 +484,13 @@ hipError_t launchPruneWaveBatch(const PruneArgs *d, const int *bi, int nBlocks, 
 // n words of p set to v by a small kernel (usable under stream capture,
 // unlike hipMemsetAsync)
 hipError_t launchFillU32(unsigned *p, unsigned v, int n, hipStream_t s);
+// Device-to-device copies of n ranges in one launch (bytes: multiples of 4).
+struct CopyRange {
+    const void *src;
+    void *dst;
+    unsigned long long bytes;
+};
+hipError_t launchCopyRanges(const CopyRange *r, int n, hipStream_t s);
 
 // ------------------------------------------- root-side consumers (p4g_anc.hip)
 // Philox4x64-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers:

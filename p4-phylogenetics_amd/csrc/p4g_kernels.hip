@@ -2385,4 +2385,30 @@ hipError_t launchFillU32(unsigned *p, unsigned v, int n, hipStream_t s)
     return hipGetLastError();
 }
 
+// blockIdx.x: the range; blockIdx.y strides over it.  16-byte moves when
+// both addresses and the size allow, else 4-byte words.
+__global__ __launch_bounds__(256) void k_copyRanges(const CopyRange *r)
+{
+    const CopyRange c = r[blockIdx.x];
+    const size_t step = (size_t)gridDim.y * 256, first = (size_t)blockIdx.y * 256 + threadIdx.x;
+    if ((((uintptr_t)c.src | (uintptr_t)c.dst | (uintptr_t)c.bytes) & 15) == 0) {
+        const uint4 *src = static_cast<const uint4 *>(c.src);
+        uint4 *dst = static_cast<uint4 *>(c.dst);
+        const size_t n = c.bytes / 16;
+        for (size_t i = first; i < n; i += step) dst[i] = src[i];
+    } else {
+        const unsigned *src = static_cast<const unsigned *>(c.src);
+        unsigned *dst = static_cast<unsigned *>(c.dst);
+        const size_t n = c.bytes / 4;
+        for (size_t i = first; i < n; i += step) dst[i] = src[i];
+    }
+}
+
+hipError_t launchCopyRanges(const CopyRange *r, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copyRanges, dim3(n, 16), dim3(256), 0, s, r);
+    return hipGetLastError();
+}
+
 }  // namespace p4g

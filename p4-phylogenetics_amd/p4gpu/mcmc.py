@@ -121,20 +121,16 @@ class Chain(object):
 
     # ------------------------------------------------------------ helpers
     def _markPath(self, tree, node, pNums=None):
-        """Mark node and its ancestors for recomputation."""
-        parts = range(tree.model.nParts) if pNums is None else pNums
-        n = node
-        while n is not None:
-            if not n.isLeaf or n is tree.root:
-                for p in parts:
-                    pf.p4_setConditionalLikelihoodsOfInternalNodePart(n.cNode, p)
-            n = n.parent
+        """Mark node and its ancestors for recomputation (one library call
+        per part list, not one per node and part)."""
+        if pNums is None:
+            pf.markPath(tree.cTree, node.nodeNum, -1)
+        else:
+            for p in pNums:
+                pf.markPath(tree.cTree, node.nodeNum, p)
 
     def _markAll(self, tree, pNum):
-        for n in tree.iterInternals():
-            pf.p4_setConditionalLikelihoodsOfInternalNodePart(n.cNode, pNum)
-        if tree.root.isLeaf:
-            pf.p4_setConditionalLikelihoodsOfInternalNodePart(tree.root.cNode, pNum)
+        pf.markAll(tree.cTree, pNum)
 
     # ---------------------------------------------------------- proposals
     def propose(self, prop):
@@ -323,8 +319,9 @@ class Chain(object):
         for pNum in self.touchedParts:
             src.model.parts[pNum].copyValsTo(dst.model.parts[pNum])
             dst.model.parts[pNum].setCStuff(dst.model)
-        pf.p4_copyCondLikes(src.cTree, dst.cTree, 1)
-        pf.p4_copyBigPDecks(src.cTree, dst.cTree, 1)
+        # the partial slots and P blocks either tree changed since the last
+        # sync: the proposal's recomputed path, not the whole tree
+        pf.syncChanged(src.cTree, dst.cTree)
         if self.touchedParts:
             pf.p4_copyModelPrams(src.cTree, dst.cTree)
             for pNum in self.touchedParts:

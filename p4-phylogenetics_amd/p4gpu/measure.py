@@ -88,11 +88,7 @@ def multi48(steps, warmup):
             n = nodes[rng.randrange(len(nodes))]
             pf.p4_setBrLen(n.cNode, n.br.len * (1.0 + 0.01 * (rng.random() - 0.5)))
             pf.p4_calculateBigPDecks(n.cNode)
-            m = n.parent
-            while m is not None:
-                for p in range(t.model.nParts):
-                    pf.p4_setConditionalLikelihoodsOfInternalNodePart(m.cNode, p)
-                m = m.parent
+            pf.markPath(t.cTree, n.parent.nodeNum, -1)
         return batch.logLikes()
 
     out = dict(workload="multi48: 48 taxa, 24 partitions (16 DNA + 8 protein, %d patterns), 4 chains, FP32"

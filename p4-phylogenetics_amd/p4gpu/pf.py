@@ -141,6 +141,9 @@ def _declare():
     _sig("p4g_getCondLikes2", I, V, I, V)
     _sig("p4g_getBigPDeck1stD", I, V, I, V)
     _sig("p4g_getBigPDeck2ndD", I, V, I, V)
+    _sig("p4g_markPath", I, V, I, I)
+    _sig("p4g_markAll", I, V, I)
+    _sig("p4g_syncChanged", I, V, V)
     _sig("p4g_cpoAccumulate", I, V, V)
     _sig("p4g_cpoGetSums", I, V, V, V)
     _sig("p4g_cpoReset", I, V)
@@ -521,6 +524,23 @@ def p4_getBigPDeck2ndD(cNode, pNum, nCat, dim):
     out = numpy.zeros((nCat, dim, dim), numpy.float64)
     _ok(lib().p4g_getBigPDeck2ndD(cNode, pNum, out.ctypes.data), "p4_getBigPDeck2ndD")
     return out
+
+
+# ---------------------------------------------------------- MCMC state sync
+def markPath(cTree, nodeNum, pNum=-1):
+    """Node nodeNum and its ancestors marked for recomputation in part pNum
+    (or every part, -1), in one call."""
+    _ok(lib().p4g_markPath(cTree, int(nodeNum), int(pNum)), "markPath")
+
+
+def markAll(cTree, pNum=-1):
+    _ok(lib().p4g_markAll(cTree, int(pNum)), "markAll")
+
+
+def syncChanged(cTreeSrc, cTreeDst):
+    """dst takes src's partials and P matrices where either changed since
+    their last sync (one copy launch for all parts)."""
+    _ok(lib().p4g_syncChanged(cTreeSrc, cTreeDst), "syncChanged")
 
 
 # ------------------------------------------------------------- CPO / LPML

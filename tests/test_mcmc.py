@@ -180,6 +180,17 @@ def test_incremental_equals_full_for_every_proposal(prec):
             v = full.calcLogLike(verbose=0)
             assert abs(v - ch.curTree.logLike) <= tol * abs(v), (g, v, ch.curTree.logLike)
             assert pf.p4_verifyIdentityOfTwoTrees(ch.curTree.cTree, ch.propTree.cTree) == 0
+            if g % 15 == 0:
+                # the sync leaves prop's partials and P equal to cur's, bit for bit
+                mp = ch.curTree.model.parts[0]
+                nP = ch.curTree.data.parts[0].nPatterns
+                for n, m2 in zip(ch.curTree.iterInternalsNoRoot(), ch.propTree.iterInternalsNoRoot()):
+                    a = pf.p4_getCondLikes(n.cNode, 0, mp.nGammaCat, mp.dim, nP)
+                    b = pf.p4_getCondLikes(m2.cNode, 0, mp.nGammaCat, mp.dim, nP)
+                    assert (a == b).all(), (g, n.nodeNum)
+                for n, m2 in zip(ch.curTree.iterNodesNoRoot(), ch.propTree.iterNodesNoRoot()):
+                    assert (pf.p4_getBigPDeck(n.cNode, 0, mp.nGammaCat, mp.dim) ==
+                            pf.p4_getBigPDeck(m2.cNode, 0, mp.nGammaCat, mp.dim)).all(), (g, n.nodeNum)
         names |= {p.name for p in m.proposals if p.nProposals[0] + p.nProposals[1] > 0}
     assert {"brLen", "nni", "compDir", "rMatrixDir", "gdasrv", "pInvar"} <= names
 
