@@ -146,6 +146,34 @@ def cpu_baseline(tree, seconds=10.0):
                 cpu_model=cpu_model())
 
 
+def cpu_rows(tree):
+    """CPU timings beside the GPU rows (--rows), one core: the branch
+    derivatives through the reference's own Pf code (P', P'', up-pass
+    partials; the per-site sums restated), and the oracle's numpy
+    restatements of the root draws and the simulation (kind 'port')."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import numpy
+    import oracle
+    f = oracle.flat_from_tree(tree, 0)
+    out = {}
+    if oracle.ref() is not None:
+        t0 = time.perf_counter()
+        oracle.ref_part_derivs(f)
+        out["branch_derivatives_ms"] = {"value": 1e3 * (time.perf_counter() - t0), "kind": "reference", "cores": 1}
+    _, _, cl = oracle.oracle_cl(f)
+    _, _, spi, _, _ = tree.data.parts[0].patterns()
+    u = numpy.random.default_rng(1).random(len(spi))
+    t0 = time.perf_counter()
+    oracle.anc_draw(f, cl[int(f.root)], spi, u)
+    out["anc_draw_all_sites_ms"] = {"value": 1e3 * (time.perf_counter() - t0), "kind": "port", "cores": 1,
+                                    "note": "root partials precomputed"}
+    Pm = oracle.oracle_pmats(f)
+    t0 = time.perf_counter()
+    oracle.simulate(f, Pm, len(spi), seed=3)
+    out["simulate_ms"] = {"value": 1e3 * (time.perf_counter() - t0), "kind": "port", "cores": 1}
+    return out
+
+
 def traffic_for(kernel):
     """HBM bytes per launch of `kernel` from profiles/traffic_het64.json
     (rocprofv3 FETCH_SIZE + WRITE_SIZE of the same kernel on this workload,
@@ -179,6 +207,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="seconds of CPU work per baseline leg (1 core, then N cores)")
+    ap.add_argument("--rows", action="store_true",
+                    help="also time the derivative, ancestral-draw, CPO and simulation paths (GPU and CPU)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (site patterns would be "
@@ -229,6 +259,10 @@ def main():
     }
     if cpu is not None:
         out["cpu_baseline"] = cpu
+    if args.rows:
+        from p4gpu import measure
+        tree.setPrecision("fp64")
+        out["rows"] = {"gpu": measure.rows(10), "cpu": cpu_rows(tree)}
     print(json.dumps(out))
 
 

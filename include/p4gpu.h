@@ -369,6 +369,13 @@ void p4g_setPmatRoute(int direct);
  * barrier per op; otherwise one wavefront per tile holds every category.
  * Partials are the same either way. */
 void p4g_setAaTuning(int categoryWaves, int minOps);
+/* LDS cap for the 4-state program's preloaded clean children (no
+ * counterpart in the reference): a dirty-path program preloads the clean
+ * children of its ops into LDS slots at launch; with bytes > 0 the
+ * workgroup's LDS stays at or below bytes (further clean children are read
+ * from HBM when used), keeping the occupancy of full programs; 0 (the
+ * default) lets preloads fill the LDS budget. */
+void p4g_setWavePreloadLds(int bytes);
 double p4g_lastKernelMicroseconds(const p4g_tree *t);
 
 /* ---- learned proposal network (no counterpart in the reference, whose

@@ -852,16 +852,16 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
 static bool buildGroupedProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids,
                                 int *nOpsOut, int *nKidsOut);
 
-// FNV-1a over everything the staged program depends on.
+// A 64-bit hash over everything the staged program depends on: one
+// multiply-xorshift step per value (it runs for every part of every tree
+// at every evaluation, so it has to be cheap).
 static uint64_t programKey(const Tree *t, int pNum, bool full)
 {
     const TreePart &tp = t->tp[pNum];
     uint64_t h = 1469598103934665603ull;
     auto mix = [&](int64_t v) {
-        for (int b = 0; b < 8; b++) {
-            h ^= (uint64_t)((v >> (8 * b)) & 0xff);
-            h *= 1099511628211ull;
-        }
+        h = (h ^ (uint64_t)v) * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 29;
     };
     mix(full);
     mix(tp.everComputed);
@@ -871,6 +871,7 @@ static uint64_t programKey(const Tree *t, int pNum, bool full)
     mix(pruneWaveVectorP(t->precision));
     mix(pruneWaveLean(t->precision));
     mix(pruneGroupWidthFor(tp.dim, t->precision));
+    mix(pruneWavePreloadSlots(tp.K, 1, t->data->nTax, 0, t->precision));
     mix(tp.layoutGen);
     mix(t->root);
     mix(t->nNodes);
@@ -879,9 +880,7 @@ static uint64_t programKey(const Tree *t, int pNum, bool full)
         const Node *nd = t->nodes[n];
         mix(((int64_t)nd->parent << 32) ^ (int64_t)(uint32_t)nd->leftChild);
         mix(((int64_t)nd->sibling << 32) ^ (int64_t)(uint32_t)nd->seqNum);
-        mix(nd->isLeaf);
-        mix(tp.dirty[n]);
-        mix(tp.pValid[n]);
+        mix(((int64_t)nd->isLeaf << 2) | ((int64_t)(tp.dirty[n] != 0) << 1) | (int64_t)(tp.pValid[n] != 0));
     }
     return h;
 }
@@ -1148,9 +1147,10 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     int nextPre[2] = {peak[0] + nJ, peak[1]};
     std::vector<int> preSlot(cleans.size(), -1);
     int nPre = 0;
+    const int preCap = pruneWavePreloadSlots(tp.K * nStream, tipLds, data->nTax, nTipKid, t->precision);
     for (size_t j = 0; j < cleans.size(); j++) {
         const int st = cleans[j].st;
-        if (nextPre[st] < maxSlotsS) {
+        if (nextPre[st] < std::min(maxSlotsS, preCap)) {
             preSlot[j] = nextPre[st]++;
             nPre++;
         }
@@ -1365,7 +1365,10 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
 }
 
 // Stage the program and root frequencies of one part (async).
-static bool stagePart(Tree *t, int pNum, bool full, int *nOps, std::vector<DeferredP> *defer = nullptr)
+// With `uploads`, the arena range the device lacks is not copied here but
+// listed for one copy launch that covers every item of a batched call.
+static bool stagePart(Tree *t, int pNum, bool full, int *nOps, std::vector<DeferredP> *defer = nullptr,
+                      std::vector<CopyRange> *uploads = nullptr)
 {
     TreePart &tp = t->tp[pNum];
     ModelPart *mp = t->model->parts[pNum];
@@ -1395,8 +1398,10 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps, std::vector<Defer
     if (!tp.rootPiOnDevice || std::memcmp(tp.rootPiUp, tp.hRootPi, piBytes) != 0) need(tp.offPi, tp.offPi + piBytes);
     if (!tp.progOnDevice) need(tp.offProg, tp.offProg + sizeof(int) * tp.progWords);
     if (hi > lo) {
-        if (!hipOk(hipMemcpyAsync(tp.dArena + lo, tp.hArena + lo, hi - lo, hipMemcpyHostToDevice, stream()),
-                   "stage inputs"))
+        if (uploads)
+            uploads->push_back({tp.dArenaHost + lo, tp.dArena + lo, (unsigned long long)(hi - lo)});
+        else if (!hipOk(hipMemcpyAsync(tp.dArena + lo, tp.hArena + lo, hi - lo, hipMemcpyHostToDevice, stream()),
+                        "stage inputs"))
             return false;
         std::memcpy(tp.rootPiUp, tp.hRootPi, piBytes);
         tp.rootPiOnDevice = true;
@@ -1686,6 +1691,8 @@ void p4g_setAaTuning(int categoryWaves, int minOps)
 {
     setAaTuning(categoryWaves, minOps);
 }
+
+void p4g_setWavePreloadLds(int bytes) { setWavePreloadLds(bytes); }
 
 double p4g_lastKernelMicroseconds(const p4g_tree *t)
 {
@@ -3319,8 +3326,11 @@ struct p4g_batch {
     int *hPmFirst = nullptr;
     int capPm = 0;
     double *hRes = nullptr;
+    CopyRange *hRanges = nullptr, *dRanges = nullptr;   // program uploads (pinned)
+    int capRanges = 0;
     ~p4g_batch()
     {
+        hostFree(hRanges);
         devFree(dArgs);
         devFree(dBlockItem);
         hostFree(hArgs);
@@ -3360,16 +3370,36 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     };
     std::vector<Item> items;
     std::vector<DeferredP> defer;
+    std::vector<CopyRange> uploads;
+    // the items' program and root-frequency uploads, in one copy launch
+    auto launchUploads = [&]() -> bool {
+        if (uploads.empty()) return true;
+        if ((int)uploads.size() > b->capRanges) {
+            hostFree(b->hRanges);
+            b->capRanges = 0;
+            const int cap = std::max<int>(2 * (int)uploads.size(), 64);
+            if (!hostAlloc(&b->hRanges, (size_t)cap, "hipHostMalloc upload ranges") ||
+                !hostDevPtr(b->hRanges, &b->dRanges, "upload ranges device address"))
+                return false;
+            b->capRanges = cap;
+        }
+        std::copy(uploads.begin(), uploads.end(), b->hRanges);
+        return hipOk(launchCopyRanges(b->dRanges, (int)uploads.size(), stream()), "launch upload kernel");
+    };
     for (size_t i = 0; i < b->trees.size(); i++) {
         Tree *t = b->trees[i];
         if (!treeReady(t)) {
             launchDeferredEach(defer);
+            (void)launchUploads();
+            (void)hipStreamSynchronize(stream());
             return 1;
         }
         for (int p = 0; p < t->nParts; p++) {
             int nOps = 0;
-            if (!stagePart(t, p, false, &nOps, &defer)) {
+            if (!stagePart(t, p, false, &nOps, &defer, &uploads)) {
                 launchDeferredEach(defer);
+                (void)launchUploads();
+                (void)hipStreamSynchronize(stream());
                 return 1;
             }
             items.push_back({t, p, nOps, (int)i});
@@ -3381,6 +3411,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         (void)hipStreamSynchronize(stream());
         return 1;
     };
+    if (!launchUploads()) return failQueued();
     if (!defer.empty()) {
         // the deferred direct-P flushes in one launch per state count, before
         // any pruning; the arrays hold the 20-state items, then the 4-state

@@ -474,6 +474,9 @@ bool pruneWaveVectorP(int precision);   // the wave kernel reads the k_pgather P
 bool pruneWaveStreams(int precision);   // the wave kernel splits the tree into two streams
 size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int nTipKid, int precision);
 int pruneWaveMaxSlots(int K, int tipLds, int nTax, int nTipKid, int precision);
+// slots a program's preloads may reach (a cap below pruneWaveMaxSlots)
+int pruneWavePreloadSlots(int K, int tipLds, int nTax, int nTipKid, int precision);
+void setWavePreloadLds(int bytes);
 hipError_t launchPruneWave(const PruneArgs &a, int precision, int nBlocks, size_t lds, hipStream_t s);
 // P^T of the program's children in program order: PP[k][j] = PT[kidP[j]][k]
 hipError_t launchPgather(const void *PT, const int *kidP, int nKidTot, int K, void *PP, int precision,

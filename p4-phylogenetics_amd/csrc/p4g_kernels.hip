@@ -2316,6 +2316,20 @@ size_t pruneWaveLdsBytes(int K, int nLds, int tipLds, int nTax, int nTipKid, int
     return waveTipBytes(tipLds, nTax) + (size_t)K * nTipKid * 16 * elt + (size_t)nLds * waveSlotBytes(K, precision);
 }
 
+// Preloads of clean children (dirty-path programs) may raise the
+// workgroup's LDS to at most this many bytes (0: the slot budget), so that
+// short programs keep the occupancy of full ones.
+static size_t g_preloadLds = 0;
+void setWavePreloadLds(int bytes) { g_preloadLds = bytes > 0 ? (size_t)bytes : 0; }
+int pruneWavePreloadSlots(int K, int tipLds, int nTax, int nTipKid, int precision)
+{
+    const int maxS = pruneWaveMaxSlots(K, tipLds, nTax, nTipKid, precision);
+    if (!g_preloadLds) return maxS;
+    const size_t fixed = pruneWaveLdsBytes(K, 0, tipLds, nTax, nTipKid, precision);
+    if (fixed >= g_preloadLds) return 0;
+    return std::min<int>(maxS, (int)((g_preloadLds - fixed) / waveSlotBytes(K, precision)));
+}
+
 int pruneWaveMaxSlots(int K, int tipLds, int nTax, int nTipKid, int precision)
 {
     const size_t fixed = pruneWaveLdsBytes(K, 0, tipLds, nTax, nTipKid, precision);

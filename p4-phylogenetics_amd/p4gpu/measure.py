@@ -18,6 +18,13 @@
     python -m p4gpu.measure prot128 [--steps 50]
     python -m p4gpu.measure multi48 [--steps 50]
     python -m p4gpu.measure het64 --variant -3 --precision fp32 [--steps 4]
+    python -m p4gpu.measure rows [--steps 10]
+    python -m p4gpu.measure mcmc --workload het64 --precision fp32 [--steps 200]
+
+  rows      het64 FP64: wall time per call of the branch derivatives, the
+            root ancestral draws, a CPO sample, a simulation and the
+            sequence summaries.
+  mcmc      Mcmc.run generations per second, 4 chains, on a workload.
 """
 import argparse
 import json
@@ -130,9 +137,62 @@ def het64(steps, variant, precision):
                 kernel_us_median=us[len(us) // 2], kernel_us_min=us[0])
 
 
+def _wall(fn, steps, warmup=2):
+    for _ in range(warmup):
+        fn()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    return (time.perf_counter() - t0) / steps
+
+
+def rows(steps):
+    """Wall time per call of the root-side and derivative paths on het64
+    (64 taxa, 50,000 sites, G4 + pInvar, FP64): every branch's d1/d2 with
+    the up-pass partials, one root ancestral draw of every site, one CPO
+    sample, one simulation of every site (sequences to the host included;
+    patterns not remade)."""
+    tree, data = synth.make_workload("het64", seed=1, precision="fp64")
+    tree.calcLogLike(verbose=0)
+    c = tree.cTree
+    cp = data.parts[0].cPart
+    n = pf.partNSites(cp)
+    out = dict(workload="het64: 64 taxa, 50,000 sites (%d patterns), FP64" % data.parts[0].nPatterns,
+               nodes=len(tree.nodes), sites=n)
+    out["branch_derivatives_ms"] = 1e3 * _wall(lambda: pf.branchDerivatives(c, -1, len(tree.nodes)), steps)
+    out["anc_draw_all_sites_ms"] = 1e3 * _wall(lambda: pf.drawAncStates(c, 0, 0, n, seed=7), steps)
+    pf.cpoReset(cp)
+    out["cpo_sample_ms"] = 1e3 * _wall(lambda: pf.cpoAccumulate(c), steps)
+    sim, sdata = synth.make_workload("het64", seed=2, precision="fp64")
+    sim.calcLogLike(verbose=0)
+    seeds = iter(range(10 ** 6))
+    out["simulate_ms"] = 1e3 * _wall(lambda: pf.p4_simulate(sim.cTree, None, next(seeds)), steps)
+    out["seq_stats_ms"] = 1e3 * _wall(lambda: pf.partSeqStats(sdata.parts[0].cPart), steps)
+    return out
+
+
+def mcmc(gens, warmup, workload, precision, nChains=4):
+    """Generations per second of Mcmc.run: nChains Metropolis-coupled
+    chains, the default proposal mix, no sampling inside the timed
+    generations."""
+    from .mcmc import Mcmc
+    tree, data = synth.make_workload(workload, seed=1, precision=precision)
+    m = Mcmc(tree, nChains=nChains, seed=1, verbose=False, sampleInterval=10 ** 9)
+    m.run(warmup, verbose=False)
+    t0 = time.perf_counter()
+    m.run(gens, verbose=False)
+    sec = time.perf_counter() - t0
+    acc = {p.name + ("" if p.pNum < 0 else ".%d" % p.pNum): round(sum(p.nAcceptances) / max(1, sum(p.nProposals)), 3)
+           for p in m.proposals}
+    return dict(workload="%s, %d chains, %s" % (workload, nChains, precision),
+                patterns=sum(p.nPatterns for p in data.parts), gens_per_s=gens / sec, ms_per_gen=1e3 * sec / gens,
+                acceptance=acc)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["prot128", "multi48", "het64"])
+    ap.add_argument("what", choices=["prot128", "multi48", "het64", "rows", "mcmc"])
+    ap.add_argument("--workload", default="het64")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--variant", type=int, default=-3)
@@ -142,6 +202,10 @@ def main(argv=None):
         res = prot128(a.steps, a.warmup)
     elif a.what == "multi48":
         res = multi48(a.steps, a.warmup)
+    elif a.what == "rows":
+        res = rows(a.steps)
+    elif a.what == "mcmc":
+        res = mcmc(a.steps, a.warmup, a.workload, a.precision)
     else:
         res = het64(a.steps, a.variant, a.precision)
     print(json.dumps(res))

@@ -78,6 +78,7 @@ def _declare():
     _sig("p4g_setPruneTuning", None, I, I, I)
     _sig("p4g_setPmatRoute", None, I)
     _sig("p4g_setAaTuning", None, I, I)
+    _sig("p4g_setWavePreloadLds", None, I)
     _sig("p4g_lastKernelMicroseconds", D, V)
     _sig("p4g_newData", V, I, I)
     _sig("p4g_freeData", None, V)
@@ -219,6 +220,12 @@ def setPmatRoute(direct):
 
 
 AA_CAT_MIN_OPS = 32
+
+
+def setWavePreloadLds(nBytes):
+    """Cap (bytes) on the 4-state kernel's LDS when a dirty-path program
+    preloads clean children; 0: no cap below the slot budget."""
+    lib().p4g_setWavePreloadLds(int(nBytes))
 
 
 def setAaTuning(categoryWaves, minOps=AA_CAT_MIN_OPS):
