@@ -1065,8 +1065,9 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
                 if (kids[ops[i].kidStart + c].kind == 0) nTipKid++;
         if (pruneWaveMaxSlots(tp.K, tipLds, data->nTax, nTipKid, t->precision) < 4) nTipKid = 0;
     }
+    const int nStaged = nTipKid;
     // LDS budget shared by all slot files of the workgroup
-    const int maxSlotsS = pruneWaveMaxSlots(tp.K * nStream, tipLds, data->nTax, nTipKid, t->precision);
+    const int maxSlotsS = pruneWaveMaxSlots(tp.K * nStream, tipLds, data->nTax, nStaged, t->precision);
     std::vector<int> top(nOps, -1);
     for (int q = nOps - 1; q >= 0; q--) {
         const int i = order[q];
@@ -1147,7 +1148,7 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     int nextPre[2] = {peak[0] + nJ, peak[1]};
     std::vector<int> preSlot(cleans.size(), -1);
     int nPre = 0;
-    const int preCap = pruneWavePreloadSlots(tp.K * nStream, tipLds, data->nTax, nTipKid, t->precision);
+    const int preCap = pruneWavePreloadSlots(tp.K * nStream, tipLds, data->nTax, nStaged, t->precision);
     for (size_t j = 0; j < cleans.size(); j++) {
         const int st = cleans[j].st;
         if (nextPre[st] < std::min(maxSlotsS, preCap)) {
@@ -1171,7 +1172,7 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     const size_t wordsW = (size_t)16 * nOps + 4 * (size_t)nSide + 3 * (size_t)nPre + (size_t)nKidTot;
     const size_t popsOff = (wordsW + 15) & ~(size_t)15;   // LeanOps, 64-byte aligned
     const size_t tipPnOff = popsOff + (size_t)16 * nOps;
-    const size_t words = tipPnOff + (size_t)nTipKid;
+    const size_t words = tipPnOff + (size_t)nStaged;
     if ((int)words > tp.progCapWords) {
         setError("internal: node program larger than its staging area");
         return false;
@@ -1250,7 +1251,10 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
             r.kind = PK_GEN;
             r.gen = q;
             r.ld = -1;
-            const bool tipsOk = nTipKid > 0;
+            // tip children: staged P^T rows in LDS, or (no room to stage
+            // them: FP64) scalar P^T and the select tree; both need the
+            // tile's tip codes in LDS
+            const bool tipsOk = tipLds != 0;
             const bool fast = w.nKids == 2 && !(w.flags & OP_ROOT) && w.ldsDst >= -1 &&
                               ((w.kind[0] == WK_TIP && tipsOk) || w.kind[0] == WK_LDS) &&
                               ((w.kind[1] == WK_TIP && tipsOk) || w.kind[1] == WK_LDS);
@@ -1264,13 +1268,21 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
                 for (int u = 0; u < 2; u++) {
                     const int c = cs[u];
                     if (w.kind[c] == WK_TIP) {
-                        if (nt >= nTipKid) {
-                            setError("internal: staged tip count");
+                        *aa[u] = w.idx[c] * 64;
+                        const size_t pb = (size_t)w.pnode[c] * tp.K * blkB;
+                        if (pb > (size_t)INT32_MAX) {
+                            setError("internal: P table too large for 32-bit offsets");
                             return false;
                         }
-                        *aa[u] = w.idx[c] * 64;
-                        *tt[u] = (int)(nt * blkB);
-                        tipPn[nt++] = w.pnode[c];
+                        *pp[u] = (int)pb;
+                        if (nTipKid > 0) {
+                            if (nt >= nTipKid) {
+                                setError("internal: staged tip count");
+                                return false;
+                            }
+                            *tt[u] = (int)(nt * blkB);
+                            tipPn[nt++] = w.pnode[c];
+                        }
                     } else {
                         *aa[u] = w.idx[c];
                         *tt[u] = w.idx[c] * 64;
@@ -1294,7 +1306,7 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     }
     tp.popsOff = (int)popsOff;
     tp.tipPnOff = (int)tipPnOff;
-    tp.nTipKid = nTipKid;
+    tp.nTipKid = nStaged;
     tp.nWOps = nOps;
     tp.nWKids = nSide;
     tp.nPre = nPre;
@@ -1303,7 +1315,7 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     tp.nKidTot = nKidTot;
     tp.nStream = nStream;
     tp.nS0 = (int)lists[0].size();
-    tp.waveLds = pruneWaveLdsBytes(tp.K * nStream, nLds, tipLds, data->nTax, nTipKid, t->precision);
+    tp.waveLds = pruneWaveLdsBytes(tp.K * nStream, nLds, tipLds, data->nTax, nStaged, t->precision);
     tp.progWords = (int)words;
     tp.nGroups = 0;
     tp.nKids = (int)kids.size();

@@ -179,7 +179,7 @@ struct LeanOp {
     int a0, a1;          // tip: code offset row * 64; LDS child: slot index
     int t0, t1;          // tip: byte offset of its staged P^T block in the category's region;
                          // LDS child: exponent index slot * 64
-    int p0, p1;          // LDS child: byte offset of its P^T block, node * K * 16 * element
+    int p0, p1;          // byte offset of the child's P^T block in global memory, node * K * 16 * element
     int dstLo, dstHi;    // result slot * K * nPatPad (in 4-state vectors)
     int pad0, pad1;
     int ld;              // LDS slot of the result, or -1

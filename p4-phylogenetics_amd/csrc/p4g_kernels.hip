@@ -1620,7 +1620,7 @@ __device__ __forceinline__ void pruneWaveBody(const PruneArgs &a, int tile, int 
     const int p = tile * 64 + lane;
     const size_t nPP = (size_t)a.nPatPad;
     const int nL = a.nLds;
-    const int tipBytes = a.tipLds ? ((a.nTax * 64 + 15) & ~15) : 0;
+    const int tipBytes = a.tipLds ? ((a.nTax * 64 + 31) & ~31) : 0;   // 32-byte aligned: FP64 rows
     // explicit address spaces keep LDS and HBM reads of a child as separate
     // ds_ / global_ instructions (one merged generic load would wait on
     // both memory counters at every child)
@@ -1974,7 +1974,7 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
     const int p = tile * 64 + lane;
     const size_t nPP = (size_t)a.nPatPad;
     const int nL = a.nLds;
-    const int tipBytes = a.tipLds ? ((a.nTax * 64 + 15) & ~15) : 0;
+    const int tipBytes = a.tipLds ? ((a.nTax * 64 + 31) & ~31) : 0;   // 32-byte aligned: FP64 rows
     const int blkB = 16 * (int)sizeof(T);                   // one P^T block
     for (int j = threadIdx.x; j < a.nAmb && j < 251; j += blockDim.x) ambL[j] = (unsigned char)(a.ambMask[j] & 15u);
     const int tipPBytes = K * a.nTipKid * blkB;
@@ -2073,6 +2073,34 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
     __builtin_amdgcn_s_waitcnt(VMCNT0);
 
     crec16 *lops = (crec16 *)(a.pops);
+    // the program's tip children read P^T rows staged in LDS, or (none
+    // staged) scalar P^T through leafVec4's select tree, with the
+    // partial-ambiguity masks from LDS (no global load in the fast loop)
+    const bool stagedTips = a.nTipKid > 0;
+    auto leafS = [&](const T *Pl, unsigned code, T *v) {
+        const bool b0 = code & 1u, b1 = code & 2u;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const T lo = b0 ? Pl[4 + s] : Pl[s];
+            const T hi = b0 ? Pl[12 + s] : Pl[8 + s];
+            v[s] = b1 ? hi : lo;
+        }
+        if (code >= 4u) {
+            if (code == TIP_ALL) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) v[s] = T(1);
+            } else {
+                const unsigned m = ambL[code - 4];
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    T x = T(0);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) x += ((m >> j) & 1u) ? Pl[j * 4 + s] : T(0);
+                    v[s] = x;
+                }
+            }
+        }
+    };
     cint_t *opsS = (cint_t *)(a.wops);
     cint_t *sideS = (cint_t *)(a.wkids);
     const int nW = a.nWOps;
@@ -2135,9 +2163,25 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
                 internalVec<4, T>(Pa, xv, 4, u);
                 internalVec<4, T>(Pb, yv, 4, w);
             } else {
-                tipRow(r[3], (unsigned)tipCodes[r[1]], u);
+                if (stagedTips) {
+                    tipRow(r[3], (unsigned)tipCodes[r[1]], u);
+                } else {
+                    cT *Pt = (cT *)(PTc + r[5]);
+                    T Pl[16];
+#pragma unroll
+                    for (int j = 0; j < 16; j++) Pl[j] = Pt[j];
+                    leafS(Pl, (unsigned)tipCodes[r[1]], u);
+                }
                 if (kind == PK_TT) {
-                    tipRow(r[4], (unsigned)tipCodes[r[2]], w);
+                    if (stagedTips) {
+                        tipRow(r[4], (unsigned)tipCodes[r[2]], w);
+                    } else {
+                        cT *Pt = (cT *)(PTc + r[6]);
+                        T Pl[16];
+#pragma unroll
+                        for (int j = 0; j < 16; j++) Pl[j] = Pt[j];
+                        leafS(Pl, (unsigned)tipCodes[r[2]], w);
+                    }
                 } else {
                     const V4 y = ldS(r[2]);
                     if (SCALE) e = slotE[r[4]];
@@ -2300,7 +2344,7 @@ bool pruneUsesWave(int dim, int K, int precision)
 constexpr int LEAN_LDS_BUDGET32 = 56 * 1024;
 constexpr int LEAN_LDS_BUDGET64 = 56 * 1024;
 
-static size_t waveTipBytes(int tipLds, int nTax) { return tipLds ? (((size_t)nTax * 64 + 15) & ~(size_t)15) : 0; }
+static size_t waveTipBytes(int tipLds, int nTax) { return tipLds ? (((size_t)nTax * 64 + 31) & ~(size_t)31) : 0; }
 
 static size_t waveSlotBytes(int K, int precision)
 {

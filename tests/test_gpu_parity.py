@@ -254,14 +254,15 @@ def test_wave_kernel_variants_match_golden(prec, tol):
     """The pair kernel, every variant of the wave kernel (P by scalar or
     vector loads, one or two streams) and the grouped kernel give the
     reference lnL (the autotuner may pick any of them)."""
-    fx = load("het64_800")
     fp32 = prec == "fp32"
     try:
-        for G in (-3, -1, -2, 0, 4):
-            pf.setPruneTuning(fp32, G, 1)
-            t = tree_from_fixture(fx, prec)
-            v = t.calcLogLike(verbose=0)
-            assert rel(v, float(fx["lnL"])) < tol, (G, v, float(fx["lnL"]))
+        for case in ("het64_800", "edge_dna_gaps_pinv"):   # the second has gaps and partial ambiguities
+            fx = load(case)
+            for G in (-3, -1, -2, 0, 4):
+                pf.setPruneTuning(fp32, G, 1)
+                t = tree_from_fixture(fx, prec)
+                v = t.calcLogLike(verbose=0)
+                assert rel(v, float(fx["lnL"])) < tol, (case, G, v, float(fx["lnL"]))
     finally:
         pf.setPruneTuning(fp32, -3, 1)   # the library default, then the tuned choice
         pf._applyTuning()

@@ -209,25 +209,28 @@ def test_prior_only_run_recovers_branch_length_prior():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("G", [-3, -1, -2, 0, 4])
-def test_incremental_equals_full_for_every_kernel_variant(G):
+@pytest.mark.parametrize("G,prec", [(-3, "fp32"), (-1, "fp32"), (-2, "fp32"), (0, "fp32"),
+                                    (4, "fp32"), (-3, "fp64"), (-1, "fp64")])
+def test_incremental_equals_full_for_every_kernel_variant(G, prec):
     """Dirty-path evaluations (preloaded clean children, the pair kernel's
     generic and fast ops, junction slots of the two-stream variant, the
     program-order P stream, the grouped kernel) equal a full recomputation,
     generation after generation."""
-    tree, _ = synth.make_workload("het64", seed=31, nSites=1200, precision="fp32")
+    fp32 = prec == "fp32"
+    tree, _ = synth.make_workload("het64", seed=31, nSites=1200, precision=prec)
     try:
-        pf.setPruneTuning(True, G, 1)
+        pf.setPruneTuning(fp32, G, 1)
         m = Mcmc(tree, nChains=2, seed=9, verbose=False)
         for g in range(60):
             m.run(1, verbose=False)
             for ch in m.chains:
                 full = ch.curTree.dupe()
-                full.precision = "fp32"
+                full.precision = prec
                 v = full.calcLogLike(verbose=0)
-                assert abs(v - ch.curTree.logLike) <= 1e-7 * abs(v), (G, g, v, ch.curTree.logLike)
+                tol = 1e-7 if fp32 else 1e-11
+                assert abs(v - ch.curTree.logLike) <= tol * abs(v), (G, g, v, ch.curTree.logLike)
     finally:
-        pf.setPruneTuning(True, -3, 1)   # the library default, then the tuned choice
+        pf.setPruneTuning(fp32, -3, 1)   # the library default, then the tuned choice
         pf._applyTuning()
 
 
