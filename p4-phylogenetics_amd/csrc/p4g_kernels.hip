@@ -1423,12 +1423,15 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
     if (k == 0) finishSiteSumWave(a, contrib, bad, tile, nTiles);
 }
 
-__global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneAaCat(PruneArgs a)
+// at most 102 VGPRs: five waves per SIMD, so prot128's 4,976 category
+// waves are resident in one round on 1,024 SIMDs (four per SIMD took two)
+__global__ __launch_bounds__(64 * WAVE_MAX_K) __attribute__((amdgpu_waves_per_eu(5))) void k_pruneAaCat(PruneArgs a)
 {
     pruneAaCatBody(a, blockIdx.x, gridDim.x);
 }
 
-__global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneAaCatBatch(const PruneArgs *args, const int *blockItem)
+__global__ __launch_bounds__(64 * WAVE_MAX_K) __attribute__((amdgpu_waves_per_eu(5))) void k_pruneAaCatBatch(
+    const PruneArgs *args, const int *blockItem)
 {
     const int item = blockItem[2 * blockIdx.x];
     const int local = blockItem[2 * blockIdx.x + 1];
