@@ -9,9 +9,9 @@
 //               Pf/p4_node.c:545-586, whose last entry is 1.0).  Uniforms
 //               come from Philox4x64-10 keyed by the seed and counted by
 //               (site, node, part), so a site's draws do not depend on the
-//               launch shape.  States go to a [node][site] byte buffer
-//               (coalesced across the wavefront); leaves also to the
-//               part's [taxon][site] sequences.
+//               launch shape.  Node states live in LDS ([node][thread] bytes;
+//               a global [node][site] buffer for trees too large for that);
+//               leaves also go to the part's [taxon][site] sequences.
 //  k_seqStats   one thread per site: the distinct states among the taxa
 //               (a bit mask), whether the site is constant, and per-taxon
 //               state counts in LDS, added to global counters once per
@@ -26,12 +26,25 @@ constexpr uint64_t SIM_CTR = 0x73696dull;       // "sim"
 constexpr uint64_t SIM_KEY_ROOT = 0x726f6f74ull; // "root"
 constexpr uint64_t SIM_KEY_NODE = 0x6e6f6465ull; // "node"
 
+// LDS: the node states of the workgroup's sites live in LDS, [node][thread]
+// bytes, so each node reads its parent's state there instead of from the
+// global buffer it wrote a moment before (a dependent round trip per node).
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_simulate(SimArgs a)
 {
-    const int c = blockIdx.x * 256 + threadIdx.x;
+    extern __shared__ uint8_t stL[];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.nChar) return;
     const int S = a.dim, K = a.K;
     const size_t nC = (size_t)a.nChar;
+    const int bd = blockDim.x, tx = threadIdx.x;
+    auto put = [&](int n, int v) {
+        if (LDS)
+            stL[n * bd + tx] = (uint8_t)v;
+        else
+            a.states[(size_t)n * nC + c] = (uint8_t)v;
+    };
+    auto get = [&](int n) -> int { return LDS ? (int)stL[n * bd + tx] : (int)a.states[(size_t)n * nC + c]; };
     int cat = 0, state = 0;
     bool inv = false;
     if (a.refDraw) {
@@ -59,13 +72,13 @@ __global__ __launch_bounds__(256) void k_simulate(SimArgs a)
         }
         inv = a.pInvar > 0.0 && uniform53(r.v[2]) < a.pInvar;
     }
-    a.states[(size_t)a.root * nC + c] = (uint8_t)state;
+    put(a.root, state);
     if (a.seqNum[a.root] >= 0) a.seq[(size_t)a.seqNum[a.root] * nC + c] = (int8_t)state;
     a.simCat[c] = inv ? -1 : cat;
     const size_t SS = (size_t)S * S;
     for (int i = 0; i < a.nPre; i++) {
         const int n = a.pre[i];
-        const int ps = a.states[(size_t)a.parent[n] * nC + c];
+        const int ps = get(a.parent[n]);
         int s = ps;
         if (!inv) {
             const double u = uniform53(
@@ -82,15 +95,23 @@ __global__ __launch_bounds__(256) void k_simulate(SimArgs a)
                 }
             }
         }
-        a.states[(size_t)n * nC + c] = (uint8_t)s;
+        put(n, s);
         if (a.seqNum[n] >= 0) a.seq[(size_t)a.seqNum[n] * nC + c] = (int8_t)s;
     }
 }
 
+constexpr int SIM_LDS_MAX = 64 * 1024;
+
 hipError_t launchSimulate(const SimArgs &a, hipStream_t s)
 {
     if (a.nChar <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_simulate, dim3((a.nChar + 255) / 256), dim3(256), 0, s, a);
+    // the largest workgroup (256, 128 or 64 sites) whose node states fit in 64 KB of LDS
+    int bd = 256;
+    while (bd > 64 && (size_t)a.nNodes * bd > SIM_LDS_MAX) bd /= 2;
+    if ((size_t)a.nNodes * bd <= SIM_LDS_MAX)
+        hipLaunchKernelGGL(k_simulate<true>, dim3((a.nChar + bd - 1) / bd), dim3(bd), (size_t)a.nNodes * bd, s, a);
+    else
+        hipLaunchKernelGGL(k_simulate<false>, dim3((a.nChar + 255) / 256), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
