@@ -63,6 +63,13 @@ int p4g_pokeSequences(p4g_part *p, const char *seqs);
 /* pf.makePatterns -> makePatterns (Pf/pfmodule.c:186-197, Pf/part.c:317-448).
  * Same first-occurrence order and counts; hashed instead of quadratic. */
 int p4g_makePatterns(p4g_part *p);
+/* makePatterns on the GPU (no separate counterpart: the device form of
+ * Pf/part.c:317-448): the same patterns, counts, first-occurrence order and
+ * site map as p4g_makePatterns, by a hash of every site column, a radix
+ * sort, a check of every column against its run's first column (a collision
+ * retries with another hash seed) and scans.  Reads the part's device
+ * sequences when a simulation left them there. */
+int p4g_makePatternsDevice(p4g_part *p);
 /* pf.setGlobalInvarSitesVec (Pf/pfmodule.c:310-320, Pf/part.c:716-848) */
 int p4g_setGlobalInvarSitesVec(p4g_part *p);
 /* Accessors of part fields the reference reads through its struct

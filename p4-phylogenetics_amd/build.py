@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "p4gpu")
 LIB = os.path.join(OUT_DIR, "libp4gpu.so")
 OBJ_DIR = os.path.join(HERE, "_build")
-SOURCES = ["p4g_gamma.hip", "p4g_kernels.hip", "p4g_api.hip", "p4g_mlp.hip", "p4g_deriv.hip", "p4g_anc.hip", "p4g_sim.hip"]
+SOURCES = ["p4g_gamma.hip", "p4g_kernels.hip", "p4g_api.hip", "p4g_mlp.hip", "p4g_deriv.hip", "p4g_anc.hip", "p4g_sim.hip", "p4g_compress.hip"]
 ARCH = os.environ.get("P4G_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -260,6 +260,8 @@ static bool checkCodes(const Part *p)
     return true;
 }
 
+static bool partSeqBuffer(Part *p);
+
 // Upload one data part in the device layout of the pruning kernel.
 static bool uploadPart(Part *p)
 {
@@ -1850,6 +1852,47 @@ int p4g_makePatterns(p4g_part *p)
     return 0;
 }
 
+// makePatterns on the GPU: the same patterns, counts and site map as
+// p4g_makePatterns (Pf/part.c:317-448), from the part's device sequences
+// (a simulation leaves them there; otherwise the host sequences go up).
+int p4g_makePatternsDevice(p4g_part *p)
+{
+    clearError();
+    if (!p) return 1;
+    if (!deviceReady()) return 1;
+    const size_t n = (size_t)p->nTax * p->nChar;
+    if (p->nChar < 1 || p->sequences.size() != n) {
+        setError("makePatternsDevice: the part has no sequences");
+        return 1;
+    }
+    hipStream_t s = stream();
+    if (!p->dSeqValid) {
+        std::vector<int8_t> h(n);
+        for (size_t i = 0; i < n; i++) h[i] = (int8_t)p->sequences[i];   // codes fit: -64..63
+        if (!partSeqBuffer(p) ||
+            !hipOk(hipMemcpyAsync(p->dSeq, h.data(), n, hipMemcpyHostToDevice, s), "copy sequences") ||
+            !hipOk(hipStreamSynchronize(s), "copy sequences"))
+            return 1;
+        p->dSeqValid = true;
+    }
+    std::vector<int8_t> pats;
+    std::vector<int> counts, sitePat;
+    int nP = 0, coll = 0;
+    const hipError_t e = compressPatterns(p->dSeq, p->nTax, p->nChar, pats, counts, sitePat, &nP, &coll);
+    if (e != hipSuccess) {
+        setError(coll >= 4 ? "makePatternsDevice: the column hash collided under four seeds"
+                           : std::string("makePatternsDevice: ") + hipGetErrorString(e));
+        return 1;
+    }
+    p->nPatterns = nP;
+    p->patterns.assign(pats.begin(), pats.end());
+    p->patternCounts = counts;
+    p->seqPosPatternIndex = sitePat;
+    p->invarSet = false;
+    p->dev.valid = false;
+    return checkCodes(p) ? 0 : 1;
+}
+
 int p4g_setGlobalInvarSitesVec(p4g_part *p)
 {
     clearError();
@@ -3256,10 +3299,7 @@ int p4g_partSeqStats(p4g_part *p, int *compCounts, int *nConstant, double *meanN
     }
     if (!p->dSeqValid) {
         std::vector<int8_t> h(n);
-        for (size_t i = 0; i < n; i++) {
-            const int c = p->sequences[i];
-            h[i] = (int8_t)(c >= 0 && c < p->dim ? c : -1);
-        }
+        for (size_t i = 0; i < n; i++) h[i] = (int8_t)p->sequences[i];   // exact codes: -64..63
         if (!partSeqBuffer(p)) return 1;
         if (!hipOk(hipMemcpyAsync(p->dSeq, h.data(), n, hipMemcpyHostToDevice, s), "copy sequences") ||
             !hipOk(hipStreamSynchronize(s), "copy sequences"))

@@ -579,6 +579,13 @@ hipError_t launchSimulate(const SimArgs &a, hipStream_t s);
 // (counts [nTax][dim], added to), and acc[0] += constant sites, acc[1] +=
 // distinct states summed over sites, acc[2] += codes outside 0..dim-1.
 hipError_t launchSeqStats(const int8_t *seq, int nTax, int nChar, int dim, int *counts, int *acc, hipStream_t s);
+
+// ------------------------------------------- pattern compression (p4g_compress.hip)
+// The distinct columns of seq [nTax][nChar] in order of first occurrence:
+// pats [nTax][nPat] (codes), counts [nPat], sitePat [nChar].  collisions:
+// hash seeds that collided (the pipeline retries up to four).
+hipError_t compressPatterns(const int8_t *dSeq, int nTax, int nChar, std::vector<int8_t> &pats,
+                            std::vector<int> &counts, std::vector<int> &sitePat, int *nPatOut, int *collisions);
 // sum[p] += 1 / like[p] for p < nPat (the CPO sums)
 hipError_t launchCpoAccum(const double *like, double *sum, int nPat, hipStream_t s);
 

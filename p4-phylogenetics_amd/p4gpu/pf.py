@@ -88,6 +88,7 @@ def _declare():
     _sig("p4g_pokeEquatesTable", I, V, c_char_p)
     _sig("p4g_pokeSequences", I, V, c_char_p)
     _sig("p4g_makePatterns", I, V)
+    _sig("p4g_makePatternsDevice", I, V)
     _sig("p4g_setGlobalInvarSitesVec", I, V)
     _sig("p4g_partNPatterns", I, V)
     _sig("p4g_partNChar", I, V)
@@ -267,6 +268,11 @@ def pokeEquatesTable(cPart, theString):
 
 def pokeSequences(cPart, theString):
     _ok(lib().p4g_pokeSequences(cPart, theString.encode()), "pokeSequences")
+
+
+def makePatternsDevice(cPart):
+    """makePatterns on the GPU: same patterns, order, counts and site map."""
+    _ok(lib().p4g_makePatternsDevice(cPart), "makePatternsDevice")
 
 
 def makePatterns(cPart):

@@ -578,7 +578,7 @@ class Tree(object):
         pf.p4_simulate(self.cTree, refTree.cTree if refTree is not None else None, int(seed))
         if calculatePatterns:
             for p in self.data.parts:
-                pf.makePatterns(p.cPart)
+                pf.makePatternsDevice(p.cPart)   # the sequences are still on the device
                 pf.setGlobalInvarSitesVec(p.cPart)
                 p.nPatterns = pf.partPatternCount(p.cPart)
         if resetSequences and self.data.alignments:
