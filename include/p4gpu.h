@@ -383,6 +383,11 @@ void p4g_setAaTuning(int categoryWaves, int minOps);
  * from HBM when used), keeping the occupancy of full programs; 0 (the
  * default) lets preloads fill the LDS budget. */
 void p4g_setWavePreloadLds(int bytes);
+/* LDS slot file of the 20-state category-wave kernel (no counterpart in the
+ * reference): its programs run in Sethi-Ullman order and forward up to
+ * maxSlots children per workgroup through LDS instead of re-reading them
+ * from HBM (default 5; 0 turns forwarding off).  Same partials either way. */
+void p4g_setAaLdsSlots(int maxSlots);
 double p4g_lastKernelMicroseconds(const p4g_tree *t);
 
 /* ---- learned proposal network (no counterpart in the reference, whose

@@ -853,6 +853,8 @@ static bool collectOps(Tree *t, int pNum, bool full, std::vector<OpRec> &ops, st
 static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids);
 static bool buildGroupedProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids,
                                 int *nOpsOut, int *nKidsOut);
+static bool buildAaProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids,
+                           int *nOpsOut, int *nKidsOut);
 
 // A 64-bit hash over everything the staged program depends on: one
 // multiply-xorshift step per value (it runs for every part of every tree
@@ -873,6 +875,7 @@ static uint64_t programKey(const Tree *t, int pNum, bool full)
     mix(pruneWaveVectorP(t->precision));
     mix(pruneWaveLean(t->precision));
     mix(pruneGroupWidthFor(tp.dim, t->precision));
+    mix(pruneAaLdsSlots(tp.K));
     mix(pruneWavePreloadSlots(tp.K, 1, t->data->nTax, 0, t->precision));
     mix(tp.layoutGen);
     mix(t->root);
@@ -906,10 +909,14 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
     for (const OpRec &op : ops)
         if (op.dst >= 0) tp.progSlots.push_back(op.dst);
     bool ok;
+    tp.aaLds = 0;
     if (tp.wave) {
         *nOpsOut = (int)ops.size();
         *nKidsOut = (int)kids.size();
         ok = buildWaveProgram(t, pNum, ops, kids);
+    } else if (pruneUsesMfma(tp.dim, tp.K, t->precision) && pruneAaCategoryWaves((int)ops.size()) &&
+               pruneAaLdsSlots(tp.K) > 0) {
+        ok = buildAaProgram(t, pNum, ops, kids, nOpsOut, nKidsOut);
     } else {
         ok = buildGroupedProgram(t, pNum, ops, kids, nOpsOut, nKidsOut);
     }
@@ -920,6 +927,137 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
         tp.progNKids = *nKidsOut;
     }
     return ok;
+}
+
+// Postorder from the root, children with the larger slot need first
+// (Sethi-Ullman), so few results are live at once; nOps entries of
+// progKids name each op's children computed in the same program.
+static bool sethiUllmanOrder(const std::vector<OpRec> &ops, const std::vector<KidRec> &kids, int nSlots,
+                             std::vector<int> &order, std::vector<int> &parentOp, std::vector<std::vector<int>> &progKids)
+{
+    const int nOps = (int)ops.size();
+    std::vector<int> opOfSlot(nSlots, -1), need(nOps, 1);
+    parentOp.assign(nOps, -1);
+    progKids.assign(nOps, {});
+    for (int i = 0; i < nOps; i++)
+        if (!(ops[i].flags & OP_ROOT)) opOfSlot[ops[i].dst] = i;
+    for (int i = 0; i < nOps; i++)
+        for (int c = 0; c < ops[i].nKids; c++) {
+            const KidRec &k = kids[ops[i].kidStart + c];
+            if (k.kind == 1 && opOfSlot[k.idx] >= 0 && opOfSlot[k.idx] < i) {
+                progKids[i].push_back(opOfSlot[k.idx]);
+                parentOp[opOfSlot[k.idx]] = i;
+            }
+        }
+    for (int i = 0; i < nOps; i++) {
+        auto &pk = progKids[i];
+        std::stable_sort(pk.begin(), pk.end(), [&](int x, int y) { return need[x] > need[y]; });
+        int nd = 1;
+        for (size_t q = 0; q < pk.size(); q++) nd = std::max(nd, need[pk[q]] + (int)q);
+        need[i] = nd;
+    }
+    order.clear();
+    std::vector<std::pair<int, size_t>> st;
+    st.push_back({nOps - 1, 0});
+    while (!st.empty()) {
+        auto &top = st.back();
+        if (top.second < progKids[top.first].size()) {
+            const int c = progKids[top.first][top.second++];
+            st.push_back({c, 0});
+        } else {
+            order.push_back(top.first);
+            st.pop_back();
+        }
+    }
+    if ((int)order.size() != nOps) {
+        // ops whose parent is not in the program run first, in postorder
+        std::vector<char> seen(nOps, 0);
+        for (int i : order) seen[i] = 1;
+        std::vector<int> rest;
+        for (int i = 0; i < nOps; i++)
+            if (!seen[i]) rest.push_back(i);
+        order.insert(order.begin(), rest.begin(), rest.end());
+        std::vector<int> pos(nOps);
+        for (int q = 0; q < nOps; q++) pos[order[q]] = q;
+        for (int i = 0; i < nOps; i++)
+            for (int c : progKids[i])
+                if (pos[c] > pos[i]) {
+                    setError("internal: program order");
+                    return false;
+                }
+    }
+    return true;
+}
+
+// The 20-state category kernel's program: the ops in Sethi-Ullman order,
+// one group each; a result whose parent is in the program also goes to an
+// LDS slot (op flags bits 8.. = slot + 1), which the parent reads as kid
+// kind 2 (KidRec.pad = the slot; idx still names the partials slot).
+// Results that find no free slot are read back from HBM.
+static bool buildAaProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids,
+                           int *nOpsOut, int *nKidsOut)
+{
+    TreePart &tp = t->tp[pNum];
+    const int nOps = (int)ops.size();
+    std::vector<int> order, parentOp;
+    std::vector<std::vector<int>> progKids;
+    if (!sethiUllmanOrder(ops, kids, tp.nSlots, order, parentOp, progKids)) return false;
+    const int maxSlots = pruneAaLdsSlots(tp.K);
+    std::vector<int> ldsOf(nOps, -1), freeList;
+    for (int sl = maxSlots - 1; sl >= 0; sl--) freeList.push_back(sl);
+    int used = 0, peak = 0;
+    for (int i : order) {
+        for (int c : progKids[i])
+            if (ldsOf[c] >= 0) {
+                freeList.push_back(ldsOf[c]);
+                used--;
+            }
+        if (parentOp[i] >= 0 && !freeList.empty()) {
+            ldsOf[i] = freeList.back();
+            freeList.pop_back();
+            used++;
+            peak = std::max(peak, used);
+        }
+    }
+    std::vector<int> opOfSlot(tp.nSlots, -1);
+    for (int i = 0; i < nOps; i++)
+        if (!(ops[i].flags & OP_ROOT)) opOfSlot[ops[i].dst] = i;
+    std::vector<GroupRec> groups;
+    std::vector<OpRec> ops2;
+    std::vector<KidRec> kids2;
+    for (int i : order) {
+        groups.push_back(GroupRec{1, 1, (int)ops2.size(), 0});
+        OpRec o = ops[i];
+        o.flags = (o.flags & 0xff) | ((ldsOf[i] + 1) << 8);
+        o.kidStart = (int)kids2.size();
+        for (int c = 0; c < ops[i].nKids; c++) {
+            KidRec k = kids[ops[i].kidStart + c];
+            if (k.kind == 1) {
+                const int j = opOfSlot[k.idx];
+                if (j >= 0 && parentOp[j] == i && ldsOf[j] >= 0) {
+                    k.kind = 2;
+                    k.pad = ldsOf[j];
+                }
+            }
+            kids2.push_back(k);
+        }
+        ops2.push_back(o);
+    }
+    const size_t words = groups.size() * 4 + ops2.size() * 4 + kids2.size() * 4;
+    if ((int)words > tp.progCapWords) {
+        setError("internal: node program larger than its staging area");
+        return false;
+    }
+    std::memcpy(tp.hProg, groups.data(), groups.size() * sizeof(GroupRec));
+    std::memcpy(tp.hProg + groups.size() * 4, ops2.data(), ops2.size() * sizeof(OpRec));
+    std::memcpy(tp.hProg + (groups.size() + ops2.size()) * 4, kids2.data(), kids2.size() * sizeof(KidRec));
+    tp.nGroups = (int)groups.size();
+    tp.nKids = (int)kids2.size();
+    tp.progWords = (int)words;
+    tp.aaLds = peak;
+    *nOpsOut = (int)ops2.size();
+    *nKidsOut = (int)kids2.size();
+    return true;
 }
 
 static bool buildGroupedProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, const std::vector<KidRec> &kids,
@@ -1359,6 +1497,7 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     a.counter = tp.counter;
     a.result = tp.dResultHost;
     a.patLikes = siteLikes ? tp.patLikes : nullptr;
+    a.nLds = tp.aaLds;   // the 20-state category program's slot file (the wave kernels set theirs below)
     if (tp.wave) {
         a.wops = reinterpret_cast<const WaveOp *>(prog);
         a.wkids = reinterpret_cast<const WaveKid *>(prog + 16 * tp.nWOps);
@@ -1707,6 +1846,8 @@ void p4g_setAaTuning(int categoryWaves, int minOps)
 }
 
 void p4g_setWavePreloadLds(int bytes) { setWavePreloadLds(bytes); }
+
+void p4g_setAaLdsSlots(int maxSlots) { setAaLdsSlots(maxSlots); }
 
 double p4g_lastKernelMicroseconds(const p4g_tree *t)
 {
@@ -3581,7 +3722,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     std::vector<std::pair<int, int>> launches;  // (first block, count) per group
     std::vector<int> groupDim, groupPrec;
     std::vector<size_t> groupLds;
-    std::vector<int> groupK, groupWave, groupStreams, groupAaCat;
+    std::vector<int> groupK, groupWave, groupStreams, groupAaCat, groupAaLds;
     while (j < nItems) {
         const int k0 = key(order[j]);
         const int first = blk;
@@ -3601,6 +3742,11 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         groupPrec.push_back(items[order[j]].t->precision);
         groupK.push_back(items[order[j]].t->model->parts[items[order[j]].p]->nCat);
         groupAaCat.push_back(pruneAaCategoryWaves(items[order[j]].nOps) ? 1 : 0);
+        {
+            int nl = 0;   // the largest slot file of the launch's 20-state category items
+            for (int q = j; q < jj; q++) nl = std::max(nl, b->hArgs[q].nLds);
+            groupAaLds.push_back(nl);
+        }
         groupWave.push_back(items[order[j]].t->tp[items[order[j]].p].wave ? 1 : 0);
         groupStreams.push_back(items[order[j]].t->tp[items[order[j]].p].nStream);
         size_t lds = 0;
@@ -3639,7 +3785,8 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         const hipError_t le = groupWave[g] ? launchPruneWaveBatch(b->dArgs, bi, launches[g].second, groupK[g],
                                                                  groupStreams[g], groupPrec[g], groupLds[g], s)
                               : pruneUsesMfma(groupDim[g], groupK[g], groupPrec[g])
-                                  ? launchPruneAaBatch(b->dArgs, bi, launches[g].second, groupK[g], groupAaCat[g] != 0, s)
+                                  ? launchPruneAaBatch(b->dArgs, bi, launches[g].second, groupK[g], groupAaCat[g] != 0,
+                                                       groupAaLds[g], s)
                                   : launchPruneBatch(b->dArgs, bi, launches[g].second, groupDim[g], groupPrec[g],
                                                      groupLds[g], s);
         if (!hipOk(le, "launch batched pruning kernel")) return failQueued();

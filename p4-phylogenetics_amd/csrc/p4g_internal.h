@@ -235,6 +235,7 @@ struct TreePart {
     int nWOps = 0, nWKids = 0, nPre = 0, nLds = 0, tipLds = 0, nKidTot = 0;
     int nStream = 1, nS0 = 0;
     int popsOff = 0;               // LeanOp records, in ints from the program start
+    int aaLds = 0;                 // LDS slots of the 20-state category program
     int tipPnOff = 0, nTipKid = 0; // staged tip children's P^T nodes
     // The staged program is reused while everything it depends on is
     // unchanged (tree structure, postorder, marks, kernel choice, layout).
@@ -463,7 +464,13 @@ size_t pruneLdsBytes(int nKids, int K, int dim, int precision);  // 0 if P stays
 bool pruneUsesMfma(int dim, int K, int precision);
 int pruneBlockPatternsFor(int dim, int K, int precision, int nOps);
 hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s);
-hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, hipStream_t s);
+hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, int nLds,
+                              hipStream_t s);
+// the category kernel's LDS slot file: the most slots a program may use
+// (0: none), and the bytes of n slots
+int pruneAaLdsSlots(int K);
+size_t pruneAaLdsBytes(int K, int nLds);
+void setAaLdsSlots(int n);
 // 4-state parts with at most WAVE_MAX_K categories use the wave-per-category
 // kernel: partials [slot][cat][pattern][state], FP32 exponents
 // [slot][cat][pattern]; 64 patterns per workgroup of 64*K threads.

@@ -1293,10 +1293,19 @@ __device__ __forceinline__ void finishSiteSumWave(const PruneArgs &a, double con
 // take the maximum over categories, so partials and exponents are those of
 // k_pruneAa bit for bit; only the FP64 root sum over categories is added
 // in another order.
+//
+// Children computed earlier in the same launch come from an LDS slot file
+// (a.nLds slots; op flags bits 8.. = the result's slot + 1, kid kind 2 =
+// read slot KidRec.pad): per slot and category wave the tile's 20 x 16
+// values in the global tile layout, and the wave's own copy of the 16
+// exponents (every wave computes the same exponents, so no wave reads
+// another's).  Only the wave that wrote a slot region reads it, in program
+// order, so no barrier is needed for it.
 __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int nTiles)
 {
     __shared__ float mx[2][WAVE_MAX_K][AA_TILE];
     __shared__ double rootL[WAVE_MAX_K][AA_TILE];
+    extern __shared__ float aaSlots[];   // [nLds][K][320] values, then [nLds][K][16] exponents
     const int tid = threadIdx.x;
     const int k = __builtin_amdgcn_readfirstlane(tid >> 6);   // category
     const int lane = tid & 63;
@@ -1311,6 +1320,7 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
     const f4v zero = {0.f, 0.f, 0.f, 0.f};
     const f4v ones = {1.f, 1.f, 1.f, 1.f};
 
+    int *slotEx = reinterpret_cast<int *>(aaSlots + (size_t)a.nLds * K * 320);
     double contrib = 0.0;
     int bad = 0;
     int par = 0;
@@ -1325,7 +1335,22 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                 const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
                 f4v v0, v1;
                 bool skip = false;
-                if (kd.kind == 0) {
+                if (kd.kind == 2) {
+                    // a child in this wave's LDS slot region
+                    const float *X = aaSlots + ((size_t)kd.pad * K + k) * 320;
+                    f4v acc0 = zero, acc1 = zero;
+#pragma unroll
+                    for (int t = 0; t < 5; t++) {
+                        const float b = X[t * 64 + lane];
+                        const float a0 = Pk[(4 * t + q) * AA_S + pl];
+                        const float a1 = pl < 4 ? Pk[(4 * t + q) * AA_S + 16 + pl] : 0.f;
+                        acc0 = mfma4(a0, b, acc0);
+                        acc1 = mfma4(a1, b, acc1);
+                    }
+                    v0 = acc0;
+                    v1 = acc1;
+                    e += slotEx[((size_t)kd.pad * K + k) * AA_TILE + pl];
+                } else if (kd.kind == 0) {
                     const unsigned code = a.tips[(size_t)kd.idx * nPatPad + p];
                     skip = (code == TIP_ALL) && c != 0;
                     if (code < (unsigned)AA_S) {
@@ -1393,6 +1418,15 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                 if (q == 0)
                     for (int r = 0; r < 4; r++) D[((k * 5 + 4) * 4 + r) * AA_TILE + pl] = pr1[r];
                 if (k == 0 && q == 0) a.scl[(size_t)op.dst * nPatPad + p] = e;
+                const int ld = (op.flags >> 8) - 1;
+                if (ld >= 0) {
+                    // the parent reads it from LDS too (bit-identical to what went to HBM)
+                    float *L = aaSlots + ((size_t)ld * K + k) * 320;
+                    for (int r = 0; r < 4; r++) L[(q * 4 + r) * AA_TILE + pl] = pr0[r];
+                    if (q == 0)
+                        for (int r = 0; r < 4; r++) L[(4 * 4 + r) * AA_TILE + pl] = pr1[r];
+                    if (q == 0) slotEx[((size_t)ld * K + k) * AA_TILE + pl] = e;
+                }
                 continue;
             }
             // root: this category's sum_s pi_s cl_s across the four lanes of
@@ -1452,6 +1486,19 @@ void setAaTuning(int catWaves, int minOps)
 
 bool pruneAaCategoryWaves(int nOps) { return g_aaCat != 0 && nOps >= g_aaCatMinOps; }
 
+// LDS slot file of the category kernel: per slot, K x (320 values + 16
+// exponents).  Five slots for K = 4 keep a workgroup at 27 KB, so five
+// workgroups (the five waves per SIMD the kernel is built for) fit a CU.
+static int g_aaLdsSlots = 5;
+constexpr size_t AA_LDS_BUDGET = 28 * 1024;
+void setAaLdsSlots(int n) { g_aaLdsSlots = n < 0 ? 0 : n; }
+int pruneAaLdsSlots(int K)
+{
+    const size_t per = (size_t)K * (320 + 16) * 4;
+    return (int)std::min<size_t>((size_t)g_aaLdsSlots, AA_LDS_BUDGET / per);
+}
+size_t pruneAaLdsBytes(int K, int nLds) { return (size_t)nLds * K * (320 + 16) * 4; }
+
 template <int KC>
 __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneAaBatch(const PruneArgs *args, const int *blockItem)
 {
@@ -1475,8 +1522,10 @@ static int aaKC(int K) { return K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : 8; }
 hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s)
 {
     if (pruneAaCategoryWaves(a.nOps)) {
-        if (a.K < 1 || a.K > WAVE_MAX_K || nBlocks * AA_TILE > a.nPatPad) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_pruneAaCat, dim3(nBlocks), dim3(64 * a.K), 0, s, a);
+        if (a.K < 1 || a.K > WAVE_MAX_K || nBlocks * AA_TILE > a.nPatPad || a.nLds < 0 ||
+            a.nLds > pruneAaLdsSlots(a.K))
+            return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_pruneAaCat, dim3(nBlocks), dim3(64 * a.K), pruneAaLdsBytes(a.K, a.nLds), s, a);
         return hipGetLastError();
     }
     if (nBlocks * AA_TILE * (PRUNE_THREADS / 64) > a.nPatPad) return hipErrorInvalidValue;
@@ -1489,12 +1538,14 @@ hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, hipStream_t s)
+hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, int nLds,
+                              hipStream_t s)
 {
     if (catWaves) {
-        // every item of the launch has exactly K categories (the batch groups by K)
-        if (K < 1 || K > WAVE_MAX_K) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_pruneAaCatBatch, dim3(nBlocks), dim3(64 * K), 0, s, d, bi);
+        // every item of the launch has exactly K categories (the batch groups
+        // by K); nLds: the largest slot file of its items
+        if (K < 1 || K > WAVE_MAX_K || nLds < 0 || nLds > pruneAaLdsSlots(K)) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_pruneAaCatBatch, dim3(nBlocks), dim3(64 * K), pruneAaLdsBytes(K, nLds), s, d, bi);
         return hipGetLastError();
     }
     switch (aaKC(K)) {

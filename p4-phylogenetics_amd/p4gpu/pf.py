@@ -79,6 +79,7 @@ def _declare():
     _sig("p4g_setPmatRoute", None, I)
     _sig("p4g_setAaTuning", None, I, I)
     _sig("p4g_setWavePreloadLds", None, I)
+    _sig("p4g_setAaLdsSlots", None, I)
     _sig("p4g_lastKernelMicroseconds", D, V)
     _sig("p4g_newData", V, I, I)
     _sig("p4g_freeData", None, V)
@@ -227,6 +228,12 @@ def setWavePreloadLds(nBytes):
     """Cap (bytes) on the 4-state kernel's LDS when a dirty-path program
     preloads clean children; 0: no cap below the slot budget."""
     lib().p4g_setWavePreloadLds(int(nBytes))
+
+
+def setAaLdsSlots(maxSlots):
+    """LDS slots the 20-state category kernel may forward children through
+    (default 5; 0: every child from HBM)."""
+    lib().p4g_setAaLdsSlots(int(maxSlots))
 
 
 def setAaTuning(categoryWaves, minOps=AA_CAT_MIN_OPS):

@@ -373,18 +373,23 @@ def test_protein_category_waves_match_tile_waves(nCat):
         assert rel(want, float(fx["lnL"])) < FP64_TOL
     res = {}
     try:
-        for cw in (False, True):
+        # tile waves; category waves with children from HBM; category waves
+        # forwarding children through the LDS slot file (the default)
+        for cw, slots in ((False, 5), (True, 0), (True, 5), (True, 2)):
             pf.setAaTuning(cw, 0)
+            pf.setAaLdsSlots(slots)
             t = tree_from_fixture(fx, "fp32")
             v = t.calcLogLike(verbose=0)
-            assert rel(v, want) < FP32_TOL, (cw, v, want)
+            assert rel(v, want) < FP32_TOL, (cw, slots, v, want)
             cls = [pf.p4_getCondLikes(n.cNode, 0, K, S, nP) for n in t.nodes if not (n.isLeaf or n is t.root)]
-            res[cw] = (v, cls)
+            res[(cw, slots)] = (v, cls)
     finally:
         pf.setAaTuning(True)
-    assert rel(res[True][0], res[False][0]) < 1e-12
-    for a, b in zip(res[True][1], res[False][1]):
-        assert numpy.array_equal(a, b)
+        pf.setAaLdsSlots(5)
+    for key in ((True, 0), (True, 5), (True, 2)):
+        assert rel(res[key][0], res[(False, 5)][0]) < 1e-12, key
+        for a, b in zip(res[key][1], res[(False, 5)][1]):
+            assert numpy.array_equal(a, b), key
 
 
 def test_batch_mixes_protein_kernels():
