@@ -1498,6 +1498,8 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     a.result = tp.dResultHost;
     a.patLikes = siteLikes ? tp.patLikes : nullptr;
     a.nLds = tp.aaLds;   // the 20-state category program's slot file (the wave kernels set theirs below)
+    a.nTax = dp->nTax;
+    a.tipLds = pruneAaTipLds(dp->nTax) ? 1 : 0;   // the 20-state category kernel's tip codes in LDS
     if (tp.wave) {
         a.wops = reinterpret_cast<const WaveOp *>(prog);
         a.wkids = reinterpret_cast<const WaveKid *>(prog + 16 * tp.nWOps);
@@ -3722,7 +3724,8 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
     std::vector<std::pair<int, int>> launches;  // (first block, count) per group
     std::vector<int> groupDim, groupPrec;
     std::vector<size_t> groupLds;
-    std::vector<int> groupK, groupWave, groupStreams, groupAaCat, groupAaLds;
+    std::vector<int> groupK, groupWave, groupStreams, groupAaCat, groupAaTip;
+    std::vector<size_t> groupAaLds;
     while (j < nItems) {
         const int k0 = key(order[j]);
         const int first = blk;
@@ -3743,9 +3746,21 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
         groupK.push_back(items[order[j]].t->model->parts[items[order[j]].p]->nCat);
         groupAaCat.push_back(pruneAaCategoryWaves(items[order[j]].nOps) ? 1 : 0);
         {
-            int nl = 0;   // the largest slot file of the launch's 20-state category items
-            for (int q = j; q < jj; q++) nl = std::max(nl, b->hArgs[q].nLds);
+            // the largest slot file and tip codes of the launch's 20-state category items
+            size_t nl = 0;
+            int allTip = 1;
+            const bool aaCat = groupAaCat.back() && pruneUsesMfma(groupDim.back(), groupK.back(), groupPrec.back());
+            for (int q = j; q < jj && aaCat; q++) {
+                const PruneArgs &pa = b->hArgs[q];
+                if (!pruneAaCatItemOk(pa)) {
+                    setError("batch: bad 20-state category program");
+                    return failQueued();
+                }
+                nl = std::max(nl, pruneAaLdsBytes(pa.K, pa.nLds, pa.tipLds ? pa.nTax : 0));
+                allTip &= pa.tipLds ? 1 : 0;
+            }
             groupAaLds.push_back(nl);
+            groupAaTip.push_back(allTip);
         }
         groupWave.push_back(items[order[j]].t->tp[items[order[j]].p].wave ? 1 : 0);
         groupStreams.push_back(items[order[j]].t->tp[items[order[j]].p].nStream);
@@ -3786,7 +3801,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
                                                                  groupStreams[g], groupPrec[g], groupLds[g], s)
                               : pruneUsesMfma(groupDim[g], groupK[g], groupPrec[g])
                                   ? launchPruneAaBatch(b->dArgs, bi, launches[g].second, groupK[g], groupAaCat[g] != 0,
-                                                       groupAaLds[g], s)
+                                                       groupAaLds[g], groupAaTip[g] != 0, s)
                                   : launchPruneBatch(b->dArgs, bi, launches[g].second, groupDim[g], groupPrec[g],
                                                      groupLds[g], s);
         if (!hipOk(le, "launch batched pruning kernel")) return failQueued();

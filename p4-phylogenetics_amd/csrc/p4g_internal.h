@@ -464,12 +464,14 @@ size_t pruneLdsBytes(int nKids, int K, int dim, int precision);  // 0 if P stays
 bool pruneUsesMfma(int dim, int K, int precision);
 int pruneBlockPatternsFor(int dim, int K, int precision, int nOps);
 hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s);
-hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, int nLds,
-                              hipStream_t s);
+hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, size_t ldsBytes,
+                              bool tipLds, hipStream_t s);
 // the category kernel's LDS slot file: the most slots a program may use
 // (0: none), and the bytes of n slots
 int pruneAaLdsSlots(int K);
-size_t pruneAaLdsBytes(int K, int nLds);
+size_t pruneAaLdsBytes(int K, int nLds, int tipTaxa);
+bool pruneAaTipLds(int nTax);
+bool pruneAaCatItemOk(const PruneArgs &a);
 void setAaLdsSlots(int n);
 // 4-state parts with at most WAVE_MAX_K categories use the wave-per-category
 // kernel: partials [slot][cat][pattern][state], FP32 exponents

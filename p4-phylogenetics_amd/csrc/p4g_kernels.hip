@@ -1300,12 +1300,26 @@ __device__ __forceinline__ void finishSiteSumWave(const PruneArgs &a, double con
 // values in the global tile layout, and the wave's own copy of the 16
 // exponents (every wave computes the same exponents, so no wave reads
 // another's).  Only the wave that wrote a slot region reads it, in program
-// order, so no barrier is needed for it.
+// order, so no barrier is needed for it.  With a.tipLds the tile's tip
+// codes ([nTax][16] bytes) are staged in LDS once.
+//
+// Stores are deferred by one op.  The vector memory counter retires loads
+// and stores in issue order, so a load issued after an op's HBM stores
+// waits for their acknowledgement as well; here each op first issues the
+// loads of its first two children (P rows, child partials, tip rows), then
+// the previous op's HBM stores, then computes, so those loads no longer
+// queue behind the stores.  A child read from HBM that is the pending
+// result flushes it first.  Every wave writes the (identical) exponents of
+// a result, so a later op reads exponents its own wave wrote.
+template <bool TIPLDS>
 __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int nTiles)
 {
     __shared__ float mx[2][WAVE_MAX_K][AA_TILE];
     __shared__ double rootL[WAVE_MAX_K][AA_TILE];
-    extern __shared__ float aaSlots[];   // [nLds][K][320] values, then [nLds][K][16] exponents
+    __shared__ int rootE[AA_TILE];
+    // [nLds + 1][K][320] values, [nLds + 1][K][16] exponents (region nLds:
+    // the staged result of an op without a slot), [nTax][16] tip codes
+    extern __shared__ float aaSlots[];
     const int tid = threadIdx.x;
     const int k = __builtin_amdgcn_readfirstlane(tid >> 6);   // category
     const int lane = tid & 63;
@@ -1320,69 +1334,193 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
     const f4v zero = {0.f, 0.f, 0.f, 0.f};
     const f4v ones = {1.f, 1.f, 1.f, 1.f};
 
-    int *slotEx = reinterpret_cast<int *>(aaSlots + (size_t)a.nLds * K * 320);
+    int *slotEx = reinterpret_cast<int *>(aaSlots + (size_t)(a.nLds + 1) * K * 320);
+    uint8_t *tipL = reinterpret_cast<uint8_t *>(slotEx + (size_t)(a.nLds + 1) * K * AA_TILE);
+    if (TIPLDS) {
+        for (int t = tid; t < a.nTax; t += blockDim.x)
+            *reinterpret_cast<uint4 *>(tipL + t * AA_TILE) =
+                *reinterpret_cast<const uint4 *>(a.tips + (size_t)t * nPatPad + (size_t)tile * AA_TILE);
+        __syncthreads();
+    }
+    // (two separate loads, not a select of pointers: a generic pointer
+    // would make every tip code a flat load waiting on both counters)
+    auto tipCode = [&](int row) -> unsigned {
+        if (TIPLDS) return (unsigned)tipL[row * AA_TILE + pl];
+        return (unsigned)a.tips[(size_t)row * nPatPad + p];
+    };
+    // a tip row of P: this lane's four states, and states 16..19 (used on
+    // q == 0 only; pr1 of the other lanes is cleared before it is used)
+    auto tipRow = [&](const float *Pk, unsigned code, f4v &v0, f4v &v1) {
+        if (code < (unsigned)AA_S) {
+            v0 = *reinterpret_cast<const f4v *>(Pk + code * AA_S + 4 * q);
+            v1 = *reinterpret_cast<const f4v *>(Pk + code * AA_S + 16);
+        } else if (code == TIP_ALL) {
+            v0 = ones;
+            v1 = ones;
+        } else {
+            const uint64_t m = a.ambMask[code - AA_S];
+            v0 = zero;
+            v1 = zero;
+            for (int j = 0; j < AA_S; j++)
+                if ((m >> j) & 1ull) {
+                    v0 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 4 * q);
+                    v1 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 16);
+                }
+        }
+    };
+    // The pending result: an op leaves its values and exponents in LDS (its
+    // slot, or region nLds) and the next op copies them to HBM after its
+    // own loads, so nothing of it occupies registers meanwhile.  Addresses
+    // are a wave-uniform base plus a 32-bit unsigned lane byte offset (the
+    // scalar-base + vector-offset form of the global instructions).
+    bool pend = false;
+    int pDst = 0, pSrc = 0;
+    const unsigned laneB = (unsigned)(q * 64 + pl) * 4u;   // state 4q, pattern pl of a [20][16] tile
+    const unsigned laneW = (unsigned)lane * 4u;            // state q (of four), pattern pl
+    const unsigned plB = (unsigned)pl * 4u;
+    // Every lane stores the same number of words (no branch on q): states
+    // 0..15 in four stores, states 16 + q in one, and the pattern's
+    // exponent (four lanes write each, the same value).
+    auto flush = [&]() {
+        const float *L = aaSlots + ((size_t)pSrc * K + k) * 320;
+        char *D = (char *)(cl + (size_t)pDst * slotFloats + (size_t)tile * tileFloats + (size_t)k * 320);
+        float v[5];
+        for (int r = 0; r < 4; r++) v[r] = L[(q * 4 + r) * AA_TILE + pl];
+        v[4] = L[256 + q * AA_TILE + pl];
+        const int ex = slotEx[((size_t)pSrc * K + k) * AA_TILE + pl];
+        for (int r = 0; r < 4; r++) *(float *)(D + 64 * r + laneB) = v[r];
+        *(float *)(D + 1024 + laneW) = v[4];
+        *(int *)((char *)(a.scl + (size_t)pDst * nPatPad + (size_t)tile * AA_TILE) + plB) = ex;
+        pend = false;
+    };
+
+    // the program is read-only for the launch: scalar loads (a vector load
+    // of a record would wait on the memory counter behind the loads and
+    // stores before it)
+    typedef int i4v __attribute__((ext_vector_type(4)));
+    typedef const __attribute__((address_space(4))) i4v ci4v;
+    ci4v *groupsC = (ci4v *)a.groups;
+    ci4v *opsC = (ci4v *)a.ops;
+    ci4v *kidsC = (ci4v *)a.kids;
+    auto kidAt = [&](int i) -> KidRec {
+        const i4v r = kidsC[i];
+        return KidRec{r.x, r.y, r.z, r.w};
+    };
+
     double contrib = 0.0;
     int bad = 0;
     int par = 0;
+    bool sawRoot = false;
     for (int g = 0; g < a.nGroups; ++g) {
-        const GroupRec gr = a.groups[g];
+        const i4v gv = groupsC[g];
+        const GroupRec gr{gv.x, gv.y, gv.z, gv.w};
         for (int o = 0; o < gr.nOps; ++o) {
-            const OpRec op = a.ops[gr.opStart + o];
+            const i4v ov = opsC[gr.opStart + o];
+            const OpRec op{ov.x, ov.y, ov.z, ov.w};
+            // phase 1: the global loads of the first two children, the same
+            // ten loads per child whatever its kind (branch-free, so no
+            // copies at a join wait on them).  A partials child loads its
+            // MFMA A operands: rows (4t + q) of P at column pl, and at
+            // column 16 + (pl & 3) (states 16..19; lanes pl >= 4 repeat
+            // them, which only fills output rows 4..15 of acc1, on lanes
+            // q > 0, cleared before use).  A tip child loads the P row of
+            // its code: entries 4q..4q+3 in R[0..3] and 16..19 in R[6..9].
+            float R[2][10];
+            int kind[2];
+            unsigned code[2];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                kind[c] = 1;
+                code[c] = 0;
+                if (c >= op.nKids) break;
+                const KidRec kd = kidAt(op.kidStart + c);
+                const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
+                const bool tip = kd.kind == 0;
+                kind[c] = kd.kind;
+                if (kd.kind == 1 && pend && kd.idx == pDst) flush();
+                const unsigned cd = tipCode(tip ? kd.idx : 0);
+                code[c] = cd;
+                const unsigned row = cd < (unsigned)AA_S ? cd * AA_S : 0u;
+                const unsigned o0 = (tip ? row + 4 * q : (unsigned)(q * AA_S + pl)) * 4u;
+                const unsigned o1 = (tip ? row + 15 : (unsigned)(q * AA_S + 16 + (pl & 3))) * 4u;
+                const int sb = tip ? 4 : 4 * AA_S * 4;   // bytes per step t
+                const char *Pb = (const char *)Pk;
+#pragma unroll
+                for (int t = 0; t < 5; t++) {
+                    R[c][t] = *(const float *)(Pb + sb * t + o0);
+                    R[c][5 + t] = *(const float *)(Pb + sb * t + o1);
+                }
+            }
+            // phase 2: the previous op's HBM stores
+            if (pend) flush();
+            // phase 3: products
             f4v pr0 = zero, pr1 = zero;
             int e = 0;
-            for (int c = 0; c < op.nKids; ++c) {
-                const KidRec kd = a.kids[op.kidStart + c];
-                const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                if (c >= op.nKids) break;
                 f4v v0, v1;
                 bool skip = false;
-                if (kd.kind == 2) {
-                    // a child in this wave's LDS slot region
-                    const float *X = aaSlots + ((size_t)kd.pad * K + k) * 320;
-                    f4v acc0 = zero, acc1 = zero;
-#pragma unroll
-                    for (int t = 0; t < 5; t++) {
-                        const float b = X[t * 64 + lane];
-                        const float a0 = Pk[(4 * t + q) * AA_S + pl];
-                        const float a1 = pl < 4 ? Pk[(4 * t + q) * AA_S + 16 + pl] : 0.f;
-                        acc0 = mfma4(a0, b, acc0);
-                        acc1 = mfma4(a1, b, acc1);
-                    }
-                    v0 = acc0;
-                    v1 = acc1;
-                    e += slotEx[((size_t)kd.pad * K + k) * AA_TILE + pl];
-                } else if (kd.kind == 0) {
-                    const unsigned code = a.tips[(size_t)kd.idx * nPatPad + p];
-                    skip = (code == TIP_ALL) && c != 0;
-                    if (code < (unsigned)AA_S) {
-                        v0 = *reinterpret_cast<const f4v *>(Pk + code * AA_S + 4 * q);
-                        v1 = q == 0 ? *reinterpret_cast<const f4v *>(Pk + code * AA_S + 16) : zero;
-                    } else if (code == TIP_ALL) {
+                if (kind[c] == 0) {
+                    const unsigned cd = code[c];
+                    if (cd < (unsigned)AA_S) {
+                        for (int r = 0; r < 4; r++) {
+                            v0[r] = R[c][r];
+                            v1[r] = R[c][6 + r];
+                        }
+                    } else if (cd == TIP_ALL) {
                         v0 = ones;
-                        v1 = q == 0 ? ones : zero;
+                        v1 = ones;
+                        skip = c != 0;
                     } else {
-                        const uint64_t m = a.ambMask[code - AA_S];
+                        const KidRec kd = kidAt(op.kidStart + c);
+                        const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
+                        const uint64_t m = a.ambMask[cd - AA_S];
                         v0 = zero;
                         v1 = zero;
                         for (int j = 0; j < AA_S; j++)
                             if ((m >> j) & 1ull) {
                                 v0 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 4 * q);
-                                if (q == 0) v1 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 16);
+                                v1 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 16);
                             }
                     }
                 } else {
-                    const float *X = cl + (size_t)kd.idx * slotFloats + (size_t)tile * tileFloats;
+                    // (each branch reads through its own pointer: a pointer
+                    // chosen between LDS and HBM would become a flat access)
+                    const KidRec kd = kidAt(op.kidStart + c);
                     f4v acc0 = zero, acc1 = zero;
+                    // (the products sit in each branch, so the HBM branch's
+                    // loads are consumed before the join and leave nothing
+                    // for the LDS branch to wait on)
+                    float b[5];
+                    if (kd.kind == 2) {
+                        const float *X = aaSlots + ((size_t)kd.pad * K + k) * 320;
+                        e += slotEx[((size_t)kd.pad * K + k) * AA_TILE + pl];
 #pragma unroll
-                    for (int t = 0; t < 5; t++) {
-                        const float b = X[(k * 5 + t) * 64 + lane];
-                        const float a0 = Pk[(4 * t + q) * AA_S + pl];
-                        const float a1 = pl < 4 ? Pk[(4 * t + q) * AA_S + 16 + pl] : 0.f;
-                        acc0 = mfma4(a0, b, acc0);
-                        acc1 = mfma4(a1, b, acc1);
+                        for (int t = 0; t < 5; t++) b[t] = X[t * 64 + lane];
+#pragma unroll
+                        for (int t = 0; t < 5; t++) {
+                            acc0 = mfma4(R[c][t], b[t], acc0);
+                            acc1 = mfma4(R[c][5 + t], b[t], acc1);
+                        }
+                    } else {
+                        const char *X =
+                            (const char *)(cl + (size_t)kd.idx * slotFloats + (size_t)tile * tileFloats + (size_t)k * 320);
+                        e += *(const int *)((const char *)(a.scl + (size_t)kd.idx * nPatPad + (size_t)tile * AA_TILE) + plB);
+#pragma unroll
+                        for (int t = 0; t < 5; t++) b[t] = *(const float *)(X + 256 * t + laneW);
+#pragma unroll
+                        for (int t = 0; t < 5; t++) {
+                            acc0 = mfma4(R[c][t], b[t], acc0);
+                            acc1 = mfma4(R[c][5 + t], b[t], acc1);
+                        }
+                        // settle this branch's loads here: left pending past the
+                        // join, they make the LDS branch wait on every
+                        // outstanding store as well
+                        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
                     }
                     v0 = acc0;
                     v1 = acc1;
-                    e += a.scl[(size_t)kd.idx * nPatPad + p];
                 }
                 if (c == 0) {
                     pr0 = v0;
@@ -1392,6 +1530,44 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                     pr1 *= v1;
                 }
             }
+            // further children (a root with three or more) load and multiply in turn
+            for (int c = 2; c < op.nKids; ++c) {
+                const KidRec kd = kidAt(op.kidStart + c);
+                const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
+                f4v v0, v1;
+                bool skip = false;
+                if (kd.kind == 0) {
+                    const unsigned code = tipCode(kd.idx);
+                    skip = code == TIP_ALL;
+                    tipRow(Pk, code, v0, v1);
+                } else {
+                    f4v acc0 = zero, acc1 = zero;
+                    float b[5];
+                    if (kd.kind == 2) {
+                        const float *X = aaSlots + ((size_t)kd.pad * K + k) * 320;
+                        e += slotEx[((size_t)kd.pad * K + k) * AA_TILE + pl];
+#pragma unroll
+                        for (int t = 0; t < 5; t++) b[t] = X[t * 64 + lane];
+                    } else {
+                        const float *X = cl + (size_t)kd.idx * slotFloats + (size_t)tile * tileFloats + (size_t)k * 320;
+                        e += a.scl[(size_t)kd.idx * nPatPad + p];
+#pragma unroll
+                        for (int t = 0; t < 5; t++) b[t] = X[t * 64 + lane];
+                    }
+#pragma unroll
+                    for (int t = 0; t < 5; t++) {
+                        acc0 = mfma4(Pk[(4 * t + q) * AA_S + pl], b[t], acc0);
+                        acc1 = mfma4(Pk[(4 * t + q) * AA_S + 16 + (pl & 3)], b[t], acc1);
+                    }
+                    v0 = acc0;
+                    v1 = acc1;
+                }
+                if (!skip) {
+                    pr0 *= v0;
+                    pr1 *= v1;
+                }
+            }
+            if (q != 0) pr1 = zero;   // rows 4..15 of the states-16..19 products
             // rescale: max over this wave's states, then over the categories
             float m = 0.f;
             for (int r = 0; r < 4; r++) m = fmaxf(m, fmaxf(pr0[r], pr1[r]));
@@ -1413,20 +1589,19 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                 e += E;
             }
             if (!(op.flags & OP_ROOT)) {
-                float *D = cl + (size_t)op.dst * slotFloats + (size_t)tile * tileFloats;
-                for (int r = 0; r < 4; r++) D[((k * 5 + q) * 4 + r) * AA_TILE + pl] = pr0[r];
-                if (q == 0)
-                    for (int r = 0; r < 4; r++) D[((k * 5 + 4) * 4 + r) * AA_TILE + pl] = pr1[r];
-                if (k == 0 && q == 0) a.scl[(size_t)op.dst * nPatPad + p] = e;
+                // into the op's slot (the parent reads it there), or region
+                // nLds; the next op copies it to HBM
                 const int ld = (op.flags >> 8) - 1;
-                if (ld >= 0) {
-                    // the parent reads it from LDS too (bit-identical to what went to HBM)
-                    float *L = aaSlots + ((size_t)ld * K + k) * 320;
-                    for (int r = 0; r < 4; r++) L[(q * 4 + r) * AA_TILE + pl] = pr0[r];
-                    if (q == 0)
-                        for (int r = 0; r < 4; r++) L[(4 * 4 + r) * AA_TILE + pl] = pr1[r];
-                    if (q == 0) slotEx[((size_t)ld * K + k) * AA_TILE + pl] = e;
+                const int dl = ld >= 0 ? ld : a.nLds;
+                float *L = aaSlots + ((size_t)dl * K + k) * 320;
+                for (int r = 0; r < 4; r++) L[(q * 4 + r) * AA_TILE + pl] = pr0[r];
+                if (q == 0) {
+                    for (int r = 0; r < 4; r++) L[(4 * 4 + r) * AA_TILE + pl] = pr1[r];
+                    slotEx[((size_t)dl * K + k) * AA_TILE + pl] = e;
                 }
+                pend = true;
+                pDst = op.dst;
+                pSrc = dl;
                 continue;
             }
             // root: this category's sum_s pi_s cl_s across the four lanes of
@@ -1434,13 +1609,13 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
             const double *pi = a.rootPi;
             uint64_t msk = ~0ull;
             if (a.rootLeafRow >= 0) {
-                const unsigned code = a.tips[(size_t)a.rootLeafRow * nPatPad + p];
+                const unsigned code = tipCode(a.rootLeafRow);
                 msk = code < (unsigned)AA_S ? (1ull << code) : code == TIP_ALL ? ~0ull : a.ambMask[code - AA_S];
             }
             double Lk = 0.0;
             for (int r = 0; r < 4; r++) {
-                const int s0 = 4 * q + r;
-                if ((msk >> s0) & 1ull) Lk += pi[s0] * (double)pr0[r];
+                const int s0i = 4 * q + r;
+                if ((msk >> s0i) & 1ull) Lk += pi[s0i] * (double)pr0[r];
                 if (q == 0 && ((msk >> (16 + r)) & 1ull)) Lk += pi[16 + r] * (double)pr1[r];
             }
             Lk += __shfl_xor(Lk, 16);
@@ -1450,27 +1625,35 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
             if (k == 0 && q == 0) {
                 double L = 0.0;
                 for (int kk = 0; kk < K; kk++) L += rootL[kk][pl];
-                siteLogLike<true>(a, L, e, p, K, pi, contrib, bad);
+                rootL[0][pl] = L;   // the site term is taken after the loop, where its
+                rootE[pl] = e;      // constants do not occupy registers across it
             }
+            sawRoot = true;
         }
     }
+    if (pend) flush();
+    if (sawRoot && k == 0 && q == 0) siteLogLike<true>(a, rootL[0][pl], rootE[pl], p, K, a.rootPi, contrib, bad);
     if (k == 0) finishSiteSumWave(a, contrib, bad, tile, nTiles);
 }
 
 // at most 102 VGPRs: five waves per SIMD, so prot128's 4,976 category
 // waves are resident in one round on 1,024 SIMDs (four per SIMD took two)
+template <bool TIPLDS>
 __global__ __launch_bounds__(64 * WAVE_MAX_K) __attribute__((amdgpu_waves_per_eu(5))) void k_pruneAaCat(PruneArgs a)
 {
-    pruneAaCatBody(a, blockIdx.x, gridDim.x);
+    pruneAaCatBody<TIPLDS>(a, blockIdx.x, gridDim.x);
 }
 
+// TIPLDS: every item of the launch stages its tip codes (items that could
+// are read from global memory by the other instance, which ignores tipLds)
+template <bool TIPLDS>
 __global__ __launch_bounds__(64 * WAVE_MAX_K) __attribute__((amdgpu_waves_per_eu(5))) void k_pruneAaCatBatch(
     const PruneArgs *args, const int *blockItem)
 {
     const int item = blockItem[2 * blockIdx.x];
     const int local = blockItem[2 * blockIdx.x + 1];
     const PruneArgs a = args[item];
-    pruneAaCatBody(a, local, a.nPatPad / AA_TILE);
+    pruneAaCatBody<TIPLDS>(a, local, a.nPatPad / AA_TILE);
 }
 
 // Category waves pay one barrier per op; short programs (dirty paths)
@@ -1487,17 +1670,30 @@ void setAaTuning(int catWaves, int minOps)
 bool pruneAaCategoryWaves(int nOps) { return g_aaCat != 0 && nOps >= g_aaCatMinOps; }
 
 // LDS slot file of the category kernel: per slot, K x (320 values + 16
-// exponents).  Five slots for K = 4 keep a workgroup at 27 KB, so five
-// workgroups (the five waves per SIMD the kernel is built for) fit a CU.
+// exponents), and one more such region for a result without a slot.  Four
+// slots and that region for K = 4 take 26 KB, and the tip codes of up to
+// 128 taxa 2 KB more, so five workgroups (the five waves per SIMD the
+// kernel is built for) fit a CU's 160 KB.
 static int g_aaLdsSlots = 5;
-constexpr size_t AA_LDS_BUDGET = 28 * 1024;
+constexpr size_t AA_LDS_BUDGET = 28 * 1024;   // slots and the staging region
 void setAaLdsSlots(int n) { g_aaLdsSlots = n < 0 ? 0 : n; }
 int pruneAaLdsSlots(int K)
 {
     const size_t per = (size_t)K * (320 + 16) * 4;
-    return (int)std::min<size_t>((size_t)g_aaLdsSlots, AA_LDS_BUDGET / per);
+    const size_t fit = AA_LDS_BUDGET / per;
+    return (int)std::min<size_t>((size_t)g_aaLdsSlots, fit > 0 ? fit - 1 : 0);
 }
-size_t pruneAaLdsBytes(int K, int nLds) { return (size_t)nLds * K * (320 + 16) * 4; }
+// the tile's tip codes go to LDS up to this many taxa (4 KB)
+constexpr int AA_TIP_LDS_MAX_TAXA = 256;
+bool pruneAaTipLds(int nTax) { return nTax <= AA_TIP_LDS_MAX_TAXA; }
+size_t pruneAaLdsBytes(int K, int nLds, int tipTaxa)
+{
+    return (size_t)(nLds + 1) * K * (320 + 16) * 4 + (size_t)tipTaxa * AA_TILE;
+}
+static bool aaLdsArgsOk(const PruneArgs &a)
+{
+    return a.nLds >= 0 && a.nLds <= pruneAaLdsSlots(a.K) && (!a.tipLds || pruneAaTipLds(a.nTax));
+}
 
 template <int KC>
 __global__ __launch_bounds__(PRUNE_THREADS) void k_pruneAaBatch(const PruneArgs *args, const int *blockItem)
@@ -1522,10 +1718,13 @@ static int aaKC(int K) { return K <= 1 ? 1 : K <= 2 ? 2 : K <= 4 ? 4 : 8; }
 hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s)
 {
     if (pruneAaCategoryWaves(a.nOps)) {
-        if (a.K < 1 || a.K > WAVE_MAX_K || nBlocks * AA_TILE > a.nPatPad || a.nLds < 0 ||
-            a.nLds > pruneAaLdsSlots(a.K))
+        if (a.K < 1 || a.K > WAVE_MAX_K || nBlocks * AA_TILE > a.nPatPad || !aaLdsArgsOk(a))
             return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_pruneAaCat, dim3(nBlocks), dim3(64 * a.K), pruneAaLdsBytes(a.K, a.nLds), s, a);
+        const size_t lds = pruneAaLdsBytes(a.K, a.nLds, a.tipLds ? a.nTax : 0);
+        if (a.tipLds)
+            hipLaunchKernelGGL(k_pruneAaCat<true>, dim3(nBlocks), dim3(64 * a.K), lds, s, a);
+        else
+            hipLaunchKernelGGL(k_pruneAaCat<false>, dim3(nBlocks), dim3(64 * a.K), lds, s, a);
         return hipGetLastError();
     }
     if (nBlocks * AA_TILE * (PRUNE_THREADS / 64) > a.nPatPad) return hipErrorInvalidValue;
@@ -1538,14 +1737,22 @@ hipError_t launchPruneAa(const PruneArgs &a, int nBlocks, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, int nLds,
-                              hipStream_t s)
+bool pruneAaCatItemOk(const PruneArgs &a) { return aaLdsArgsOk(a); }
+
+hipError_t launchPruneAaBatch(const PruneArgs *d, const int *bi, int nBlocks, int K, bool catWaves, size_t ldsBytes,
+                              bool tipLds, hipStream_t s)
 {
     if (catWaves) {
         // every item of the launch has exactly K categories (the batch groups
-        // by K); nLds: the largest slot file of its items
-        if (K < 1 || K > WAVE_MAX_K || nLds < 0 || nLds > pruneAaLdsSlots(K)) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_pruneAaCatBatch, dim3(nBlocks), dim3(64 * K), pruneAaLdsBytes(K, nLds), s, d, bi);
+        // by K); ldsBytes: the largest slot file and tip codes of its items;
+        // tipLds: every item stages its tip codes
+        if (K < 1 || K > WAVE_MAX_K ||
+            ldsBytes > pruneAaLdsBytes(K, pruneAaLdsSlots(K), AA_TIP_LDS_MAX_TAXA))
+            return hipErrorInvalidValue;
+        if (tipLds)
+            hipLaunchKernelGGL(k_pruneAaCatBatch<true>, dim3(nBlocks), dim3(64 * K), ldsBytes, s, d, bi);
+        else
+            hipLaunchKernelGGL(k_pruneAaCatBatch<false>, dim3(nBlocks), dim3(64 * K), ldsBytes, s, d, bi);
         return hipGetLastError();
     }
     switch (aaKC(K)) {
