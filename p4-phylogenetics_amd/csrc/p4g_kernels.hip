@@ -1305,12 +1305,13 @@ __device__ __forceinline__ void finishSiteSumWave(const PruneArgs &a, double con
 //
 // Stores are deferred by one op.  The vector memory counter retires loads
 // and stores in issue order, so a load issued after an op's HBM stores
-// waits for their acknowledgement as well; here each op first issues the
-// loads of its first two children (P rows, child partials, tip rows), then
-// the previous op's HBM stores, then computes, so those loads no longer
-// queue behind the stores.  A child read from HBM that is the pending
-// result flushes it first.  Every wave writes the (identical) exponents of
-// a result, so a later op reads exponents its own wave wrote.
+// waits for their acknowledgement as well.  Here the P rows (and tip rows)
+// of an op's first two children, which depend on the program only, are
+// loaded while the op before it finishes; the op then issues the previous
+// op's HBM stores (from LDS, where each result is staged), reads its
+// children's partials (LDS, or HBM after those stores) and computes, so no
+// load queues behind a store.  Every wave writes the (identical) exponents
+// of a result, so a later op reads exponents its own wave wrote.
 template <bool TIPLDS>
 __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int nTiles)
 {
@@ -1380,7 +1381,9 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
     const unsigned plB = (unsigned)pl * 4u;
     // Every lane stores the same number of words (no branch on q): states
     // 0..15 in four stores, states 16 + q in one, and the pattern's
-    // exponent (four lanes write each, the same value).
+    // exponent (four lanes write each, the same value).  The stores are
+    // non-temporal: the launch's partials stream through L2 once, and
+    // would otherwise evict the P matrices every op gathers from there.
     auto flush = [&]() {
         const float *L = aaSlots + ((size_t)pSrc * K + k) * 320;
         char *D = (char *)(cl + (size_t)pDst * slotFloats + (size_t)tile * tileFloats + (size_t)k * 320);
@@ -1388,9 +1391,9 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
         for (int r = 0; r < 4; r++) v[r] = L[(q * 4 + r) * AA_TILE + pl];
         v[4] = L[256 + q * AA_TILE + pl];
         const int ex = slotEx[((size_t)pSrc * K + k) * AA_TILE + pl];
-        for (int r = 0; r < 4; r++) *(float *)(D + 64 * r + laneB) = v[r];
-        *(float *)(D + 1024 + laneW) = v[4];
-        *(int *)((char *)(a.scl + (size_t)pDst * nPatPad + (size_t)tile * AA_TILE) + plB) = ex;
+        for (int r = 0; r < 4; r++) __builtin_nontemporal_store(v[r], (float *)(D + 64 * r + laneB));
+        __builtin_nontemporal_store(v[4], (float *)(D + 1024 + laneW));
+        __builtin_nontemporal_store(ex, (int *)((char *)(a.scl + (size_t)pDst * nPatPad + (size_t)tile * AA_TILE) + plB));
         pend = false;
     };
 
@@ -1407,53 +1410,60 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
         return KidRec{r.x, r.y, r.z, r.w};
     };
 
+    // The first two children of an op: the same ten global loads per child
+    // whatever its kind (branch-free, so no copies at a join wait on
+    // them).  A partials child loads its MFMA A operands: rows (4t + q) of
+    // P at column pl, and at column 16 + (pl & 3) (states 16..19; lanes
+    // pl >= 4 repeat them, which only fills output rows 4..15 of acc1, on
+    // lanes q > 0, cleared before use).  A tip child loads the P row of its
+    // code: entries 4q..4q+3 in R[0..3] and 16..19 in R[6..9].  These loads
+    // depend on the program only, so an op's are issued while the op
+    // before it finishes (after its products, before its barrier).
+    float R[2][10];
+    int kind[2];
+    unsigned code[2];
+    auto issueLoads = [&](const OpRec &op) {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            kind[c] = 1;
+            code[c] = 0;
+            if (c >= op.nKids) break;
+            const KidRec kd = kidAt(op.kidStart + c);
+            const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
+            const bool tip = kd.kind == 0;
+            kind[c] = kd.kind;
+            const unsigned cd = tipCode(tip ? kd.idx : 0);
+            code[c] = cd;
+            const unsigned row = cd < (unsigned)AA_S ? cd * AA_S : 0u;
+            const unsigned o0 = (tip ? row + 4 * q : (unsigned)(q * AA_S + pl)) * 4u;
+            const unsigned o1 = (tip ? row + 15 : (unsigned)(q * AA_S + 16 + (pl & 3))) * 4u;
+            const int sb = tip ? 4 : 4 * AA_S * 4;   // bytes per step t
+            const char *Pb = (const char *)Pk;
+#pragma unroll
+            for (int t = 0; t < 5; t++) {
+                R[c][t] = *(const float *)(Pb + sb * t + o0);
+                R[c][5 + t] = *(const float *)(Pb + sb * t + o1);
+            }
+        }
+    };
+    auto opAt = [&](int i) -> OpRec {
+        const i4v r = opsC[i];
+        return OpRec{r.x, r.y, r.z, r.w};
+    };
+
     double contrib = 0.0;
     int bad = 0;
     int par = 0;
     bool sawRoot = false;
-    for (int g = 0; g < a.nGroups; ++g) {
-        const i4v gv = groupsC[g];
-        const GroupRec gr{gv.x, gv.y, gv.z, gv.w};
-        for (int o = 0; o < gr.nOps; ++o) {
-            const i4v ov = opsC[gr.opStart + o];
-            const OpRec op{ov.x, ov.y, ov.z, ov.w};
-            // phase 1: the global loads of the first two children, the same
-            // ten loads per child whatever its kind (branch-free, so no
-            // copies at a join wait on them).  A partials child loads its
-            // MFMA A operands: rows (4t + q) of P at column pl, and at
-            // column 16 + (pl & 3) (states 16..19; lanes pl >= 4 repeat
-            // them, which only fills output rows 4..15 of acc1, on lanes
-            // q > 0, cleared before use).  A tip child loads the P row of
-            // its code: entries 4q..4q+3 in R[0..3] and 16..19 in R[6..9].
-            float R[2][10];
-            int kind[2];
-            unsigned code[2];
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-                kind[c] = 1;
-                code[c] = 0;
-                if (c >= op.nKids) break;
-                const KidRec kd = kidAt(op.kidStart + c);
-                const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
-                const bool tip = kd.kind == 0;
-                kind[c] = kd.kind;
-                if (kd.kind == 1 && pend && kd.idx == pDst) flush();
-                const unsigned cd = tipCode(tip ? kd.idx : 0);
-                code[c] = cd;
-                const unsigned row = cd < (unsigned)AA_S ? cd * AA_S : 0u;
-                const unsigned o0 = (tip ? row + 4 * q : (unsigned)(q * AA_S + pl)) * 4u;
-                const unsigned o1 = (tip ? row + 15 : (unsigned)(q * AA_S + 16 + (pl & 3))) * 4u;
-                const int sb = tip ? 4 : 4 * AA_S * 4;   // bytes per step t
-                const char *Pb = (const char *)Pk;
-#pragma unroll
-                for (int t = 0; t < 5; t++) {
-                    R[c][t] = *(const float *)(Pb + sb * t + o0);
-                    R[c][5 + t] = *(const float *)(Pb + sb * t + o1);
-                }
-            }
-            // phase 2: the previous op's HBM stores
+    // the ops of all groups are contiguous, in program order (root last)
+    const int nOps = a.nOps;
+    OpRec op = opAt(0);
+    if (nOps > 0) issueLoads(op);
+    for (int i = 0; i < nOps; ++i) {
+        {
+            // the previous op's HBM stores, behind this op's loads
             if (pend) flush();
-            // phase 3: products
+            // products
             f4v pr0 = zero, pr1 = zero;
             int e = 0;
 #pragma unroll
@@ -1568,6 +1578,13 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                 }
             }
             if (q != 0) pr1 = zero;   // rows 4..15 of the states-16..19 products
+            // the next op's loads go out now (R is free), so their latency
+            // overlaps this op's barrier, rescale and result write
+            const OpRec cur = op;
+            if (i + 1 < nOps) {
+                op = opAt(i + 1);
+                issueLoads(op);
+            }
             // rescale: max over this wave's states, then over the categories
             float m = 0.f;
             for (int r = 0; r < 4; r++) m = fmaxf(m, fmaxf(pr0[r], pr1[r]));
@@ -1588,10 +1605,10 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                 }
                 e += E;
             }
-            if (!(op.flags & OP_ROOT)) {
+            if (!(cur.flags & OP_ROOT)) {
                 // into the op's slot (the parent reads it there), or region
                 // nLds; the next op copies it to HBM
-                const int ld = (op.flags >> 8) - 1;
+                const int ld = (cur.flags >> 8) - 1;
                 const int dl = ld >= 0 ? ld : a.nLds;
                 float *L = aaSlots + ((size_t)dl * K + k) * 320;
                 for (int r = 0; r < 4; r++) L[(q * 4 + r) * AA_TILE + pl] = pr0[r];
@@ -1600,7 +1617,7 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                     slotEx[((size_t)dl * K + k) * AA_TILE + pl] = e;
                 }
                 pend = true;
-                pDst = op.dst;
+                pDst = cur.dst;
                 pSrc = dl;
                 continue;
             }
