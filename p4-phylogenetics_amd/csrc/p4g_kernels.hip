@@ -1422,14 +1422,15 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
     float R[2][10];
     int kind[2];
     unsigned code[2];
+    // An op with one child loads that child twice: every op issues the same
+    // twenty loads, so the wait counts the compiler derives do not depend
+    // on the path (a shorter path would lower them for all).
     auto issueLoads = [&](const OpRec &op) {
+        ci4v *kp = kidsC + op.kidStart;
 #pragma unroll
         for (int c = 0; c < 2; c++) {
-            kind[c] = 1;
-            code[c] = 0;
-            if (c >= op.nKids) break;
-            const KidRec kd = kidAt(op.kidStart + c);
-            const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
+            const i4v kr = kp[c < op.nKids ? c : 0];
+            const KidRec kd{kr.x, kr.y, kr.z, kr.w};
             const bool tip = kd.kind == 0;
             kind[c] = kd.kind;
             const unsigned cd = tipCode(tip ? kd.idx : 0);
@@ -1438,7 +1439,7 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
             const unsigned o0 = (tip ? row + 4 * q : (unsigned)(q * AA_S + pl)) * 4u;
             const unsigned o1 = (tip ? row + 15 : (unsigned)(q * AA_S + 16 + (pl & 3))) * 4u;
             const int sb = tip ? 4 : 4 * AA_S * 4;   // bytes per step t
-            const char *Pb = (const char *)Pk;
+            const char *Pb = (const char *)PT + (size_t)(unsigned)(kd.pnode * K + k) * (AA_S * AA_S * 4);
 #pragma unroll
             for (int t = 0; t < 5; t++) {
                 R[c][t] = *(const float *)(Pb + sb * t + o0);
@@ -1456,13 +1457,21 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
     int par = 0;
     bool sawRoot = false;
     // the ops of all groups are contiguous, in program order (root last)
+    // Every op flushes a pending result (a fixed number of stores on every
+    // path, for the same reason).  The first op has none: it flushes the
+    // staging region (unwritten) to its own destination, which its real
+    // result overwrites one op later, in order, before anything reads it.
+    // Programs here have two or more ops, the root last.
     const int nOps = a.nOps;
     OpRec op = opAt(0);
-    if (nOps > 0) issueLoads(op);
+    pend = true;
+    pDst = op.dst;
+    pSrc = a.nLds;
+    issueLoads(op);
     for (int i = 0; i < nOps; ++i) {
         {
             // the previous op's HBM stores, behind this op's loads
-            if (pend) flush();
+            flush();
             // products
             f4v pr0 = zero, pr1 = zero;
             int e = 0;
@@ -1470,29 +1479,33 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
             for (int c = 0; c < 2; c++) {
                 if (c >= op.nKids) break;
                 f4v v0, v1;
-                bool skip = false;
                 if (kind[c] == 0) {
+                    // a one-state code: its P row; all states: 1 (a product
+                    // with 1.0 leaves the other factor's bits unchanged, as
+                    // skipping it would); no branch on the lane's code
                     const unsigned cd = code[c];
-                    if (cd < (unsigned)AA_S) {
-                        for (int r = 0; r < 4; r++) {
-                            v0[r] = R[c][r];
-                            v1[r] = R[c][6 + r];
+                    const bool all = cd == TIP_ALL;
+                    for (int r = 0; r < 4; r++) {
+                        v0[r] = all ? 1.f : R[c][r];
+                        v1[r] = all ? 1.f : R[c][6 + r];
+                    }
+                    // partial ambiguity (rare): the sum of its states' P rows
+                    const bool amb = cd >= (unsigned)AA_S && !all;
+                    if (__builtin_amdgcn_ballot_w64(amb)) {
+                        if (amb) {
+                            const KidRec kd = kidAt(op.kidStart + c);
+                            const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
+                            const uint64_t m = a.ambMask[cd - AA_S];
+                            v0 = zero;
+                            v1 = zero;
+                            for (int j = 0; j < AA_S; j++)
+                                if ((m >> j) & 1ull) {
+                                    v0 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 4 * q);
+                                    v1 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 16);
+                                }
                         }
-                    } else if (cd == TIP_ALL) {
-                        v0 = ones;
-                        v1 = ones;
-                        skip = c != 0;
-                    } else {
-                        const KidRec kd = kidAt(op.kidStart + c);
-                        const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
-                        const uint64_t m = a.ambMask[cd - AA_S];
-                        v0 = zero;
-                        v1 = zero;
-                        for (int j = 0; j < AA_S; j++)
-                            if ((m >> j) & 1ull) {
-                                v0 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 4 * q);
-                                v1 += *reinterpret_cast<const f4v *>(Pk + j * AA_S + 16);
-                            }
+                        // settle the rare path's loads here (see the HBM branch below)
+                        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
                     }
                 } else {
                     // (each branch reads through its own pointer: a pointer
@@ -1504,8 +1517,9 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                     // for the LDS branch to wait on)
                     float b[5];
                     if (kd.kind == 2) {
-                        const float *X = aaSlots + ((size_t)kd.pad * K + k) * 320;
-                        e += slotEx[((size_t)kd.pad * K + k) * AA_TILE + pl];
+                        const unsigned sr = (unsigned)(kd.pad * K + k);   // slot region
+                        const float *X = aaSlots + sr * 320u;
+                        e += slotEx[sr * AA_TILE + pl];
 #pragma unroll
                         for (int t = 0; t < 5; t++) b[t] = X[t * 64 + lane];
 #pragma unroll
@@ -1526,7 +1540,9 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                         }
                         // settle this branch's loads here: left pending past the
                         // join, they make the LDS branch wait on every
-                        // outstanding store as well
+                        // outstanding store as well (the wait counter is
+                        // merged across paths conservatively, so every rare
+                        // path that loads ends with such a wait)
                         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
                     }
                     v0 = acc0;
@@ -1535,7 +1551,7 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                 if (c == 0) {
                     pr0 = v0;
                     pr1 = v1;
-                } else if (!skip) {
+                } else {
                     pr0 *= v0;
                     pr1 *= v1;
                 }
@@ -1576,6 +1592,7 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                     pr0 *= v0;
                     pr1 *= v1;
                 }
+                __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): a rare path, settled here
             }
             if (q != 0) pr1 = zero;   // rows 4..15 of the states-16..19 products
             // the next op's loads go out now (R is free), so their latency
@@ -1646,6 +1663,7 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                 rootE[pl] = e;      // constants do not occupy registers across it
             }
             sawRoot = true;
+            __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the root's loads, settled here
         }
     }
     if (pend) flush();
@@ -1684,7 +1702,7 @@ void setAaTuning(int catWaves, int minOps)
     g_aaCatMinOps = minOps < 0 ? 0 : minOps;
 }
 
-bool pruneAaCategoryWaves(int nOps) { return g_aaCat != 0 && nOps >= g_aaCatMinOps; }
+bool pruneAaCategoryWaves(int nOps) { return g_aaCat != 0 && nOps >= std::max(2, g_aaCatMinOps); }
 
 // LDS slot file of the category kernel: per slot, K x (320 values + 16
 // exponents), and one more such region for a result without a slot.  Four
