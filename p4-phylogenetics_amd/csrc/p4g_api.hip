@@ -910,6 +910,7 @@ static bool buildProgram(Tree *t, int pNum, bool full, int *nOpsOut, int *nKidsO
         if (op.dst >= 0) tp.progSlots.push_back(op.dst);
     bool ok;
     tp.aaLds = 0;
+    tp.aaRecOff = -1;
     if (tp.wave) {
         *nOpsOut = (int)ops.size();
         *nKidsOut = (int)kids.size();
@@ -989,6 +990,37 @@ static bool sethiUllmanOrder(const std::vector<OpRec> &ops, const std::vector<Ki
     return true;
 }
 
+// Fused op records of a 20-state program (AaRec), appended after its kid
+// records: the category kernel reads one per op.
+static bool appendAaRecs(TreePart &tp, const std::vector<OpRec> &ops2, const std::vector<KidRec> &kids2,
+                         size_t *words)
+{
+    const size_t need = *words + ops2.size() * AA_REC_WORDS;
+    if ((int)need > tp.progCapWords) {
+        setError("internal: node program larger than its staging area");
+        return false;
+    }
+    int *r = tp.hProg + *words;
+    for (size_t i = 0; i < ops2.size(); i++, r += AA_REC_WORDS) {
+        const OpRec &o = ops2[i];
+        std::memset(r, 0, sizeof(int) * AA_REC_WORDS);
+        r[0] = o.dst;
+        r[1] = o.flags;
+        r[2] = o.nKids;
+        r[3] = o.kidStart;
+        for (int c = 0; c < 2 && c < o.nKids; c++) {
+            const KidRec &k = kids2[o.kidStart + c];
+            r[4 + 4 * c] = k.kind;
+            r[5 + 4 * c] = k.idx;
+            r[6 + 4 * c] = k.pnode;
+            r[7 + 4 * c] = k.pad;
+        }
+    }
+    tp.aaRecOff = (int)*words;
+    *words = need;
+    return true;
+}
+
 // The 20-state category kernel's program: the ops in Sethi-Ullman order,
 // one group each; a result whose parent is in the program also goes to an
 // LDS slot (op flags bits 8.. = slot + 1), which the parent reads as kid
@@ -1051,9 +1083,11 @@ static bool buildAaProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, con
     std::memcpy(tp.hProg, groups.data(), groups.size() * sizeof(GroupRec));
     std::memcpy(tp.hProg + groups.size() * 4, ops2.data(), ops2.size() * sizeof(OpRec));
     std::memcpy(tp.hProg + (groups.size() + ops2.size()) * 4, kids2.data(), kids2.size() * sizeof(KidRec));
+    size_t allWords = words;
+    if (!appendAaRecs(tp, ops2, kids2, &allWords)) return false;
     tp.nGroups = (int)groups.size();
     tp.nKids = (int)kids2.size();
-    tp.progWords = (int)words;
+    tp.progWords = (int)allWords;
     tp.aaLds = peak;
     *nOpsOut = (int)ops2.size();
     *nKidsOut = (int)kids2.size();
@@ -1113,9 +1147,11 @@ static bool buildGroupedProgram(Tree *t, int pNum, const std::vector<OpRec> &ops
     std::memcpy(tp.hProg, groups.data(), groups.size() * sizeof(GroupRec));
     std::memcpy(tp.hProg + groups.size() * 4, ops2.data(), ops2.size() * sizeof(OpRec));
     std::memcpy(tp.hProg + (groups.size() + ops2.size()) * 4, kids2.data(), kids2.size() * sizeof(KidRec));
+    size_t allWords = words;
+    if (pruneUsesMfma(tp.dim, tp.K, t->precision) && !appendAaRecs(tp, ops2, kids2, &allWords)) return false;
     tp.nGroups = (int)groups.size();
     tp.nKids = (int)kids2.size();
-    tp.progWords = (int)words;
+    tp.progWords = (int)allWords;
     *nOpsOut = (int)ops2.size();
     *nKidsOut = (int)kids2.size();
     return true;
@@ -1500,6 +1536,7 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     a.nLds = tp.aaLds;   // the 20-state category program's slot file (the wave kernels set theirs below)
     a.nTax = dp->nTax;
     a.tipLds = pruneAaTipLds(dp->nTax) ? 1 : 0;   // the 20-state category kernel's tip codes in LDS
+    a.aaRec = tp.aaRecOff >= 0 ? prog + tp.aaRecOff : nullptr;
     if (tp.wave) {
         a.wops = reinterpret_cast<const WaveOp *>(prog);
         a.wkids = reinterpret_cast<const WaveKid *>(prog + 16 * tp.nWOps);

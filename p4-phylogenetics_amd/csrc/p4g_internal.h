@@ -148,6 +148,16 @@ struct KidRec {
 };
 constexpr int OP_ROOT = 1;
 
+// The 20-state category kernel's fused op record: the op and its first two
+// children in one 64-byte line, [dst, flags, nKids, kidStart, then per
+// child (kind, idx, pnode, pad)], four words spare.  Written after the kid
+// records of that program; the kernel reads it one op ahead.
+constexpr int AA_REC_WORDS = 16;
+struct AaRec {
+    int dst, flags, nKids, kidStart;
+    int kind[2], idx[2], pnode[2], slot[2];
+};
+
 // Program of the wave-per-category DNA kernel (k_pruneWave): one 16-int
 // record per node, in execution order, children inline (up to 3; more in
 // a side array of 4-int records).  A child is a tip row, a slot of the
@@ -236,6 +246,7 @@ struct TreePart {
     int nStream = 1, nS0 = 0;
     int popsOff = 0;               // LeanOp records, in ints from the program start
     int aaLds = 0;                 // LDS slots of the 20-state category program
+    int aaRecOff = -1;             // word offset of its fused op records in the program, or -1
     int tipPnOff = 0, nTipKid = 0; // staged tip children's P^T nodes
     // The staged program is reused while everything it depends on is
     // unchanged (tree structure, postorder, marks, kernel choice, layout).
@@ -384,6 +395,7 @@ struct PruneArgs {
     int nTipKid;
     const int *tipPn;
     int nAmb;                // entries of ambMask
+    const int *aaRec;        // 20-state category kernel: fused op records, [nOps][AA_REC_WORDS]
 };
 
 // ---------------------------------------------- branch-length derivatives

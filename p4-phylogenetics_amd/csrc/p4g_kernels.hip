@@ -1397,59 +1397,82 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
         pend = false;
     };
 
-    // the program is read-only for the launch: scalar loads (a vector load
-    // of a record would wait on the memory counter behind the loads and
-    // stores before it)
+    // The program: op i's fused record (AaRec, 16 words) is loaded one op
+    // ahead, lanes 0..11 one word each (a single vector load into one
+    // register), and taken apart with readlane when the op before it
+    // finishes: a scalar load there would wait out a scalar-cache miss on
+    // most ops.  Children past the second (a root with three or more) come
+    // from the kid records.
     typedef int i4v __attribute__((ext_vector_type(4)));
     typedef const __attribute__((address_space(4))) i4v ci4v;
-    ci4v *groupsC = (ci4v *)a.groups;
-    ci4v *opsC = (ci4v *)a.ops;
     ci4v *kidsC = (ci4v *)a.kids;
     auto kidAt = [&](int i) -> KidRec {
         const i4v r = kidsC[i];
         return KidRec{r.x, r.y, r.z, r.w};
     };
+    const int *recG = a.aaRec;
+    auto recLoad = [&](int i) -> int { return lane < 12 ? recG[(size_t)i * AA_REC_WORDS + lane] : 0; };
+    auto recTake = [&](int v) -> AaRec {
+        AaRec r;
+        r.dst = __builtin_amdgcn_readlane(v, 0);
+        r.flags = __builtin_amdgcn_readlane(v, 1);
+        r.nKids = __builtin_amdgcn_readlane(v, 2);
+        r.kidStart = __builtin_amdgcn_readlane(v, 3);
+        for (int c = 0; c < 2; c++) {
+            r.kind[c] = __builtin_amdgcn_readlane(v, 4 + 4 * c);
+            r.idx[c] = __builtin_amdgcn_readlane(v, 5 + 4 * c);
+            r.pnode[c] = __builtin_amdgcn_readlane(v, 6 + 4 * c);
+            r.slot[c] = __builtin_amdgcn_readlane(v, 7 + 4 * c);
+        }
+        return r;
+    };
 
-    // The first two children of an op: the same ten global loads per child
-    // whatever its kind (branch-free, so no copies at a join wait on
-    // them).  A partials child loads its MFMA A operands: rows (4t + q) of
-    // P at column pl, and at column 16 + (pl & 3) (states 16..19; lanes
-    // pl >= 4 repeat them, which only fills output rows 4..15 of acc1, on
-    // lanes q > 0, cleared before use).  A tip child loads the P row of its
-    // code: entries 4q..4q+3 in R[0..3] and 16..19 in R[6..9].  These loads
-    // depend on the program only, so an op's are issued while the op
-    // before it finishes (after its products, before its barrier).
+    // The first two children of an op.  A partials child loads its MFMA A
+    // operands: rows (4t + q) of P at column pl, and at column 16 + (pl & 3)
+    // (states 16..19; lanes pl >= 4 repeat them, which only fills output
+    // rows 4..15 of acc1, on lanes q > 0, cleared before use).  A tip child
+    // loads the P row of its code: entries 4q..4q+3 in R[0..3] and 16..19
+    // in R[6..9].  These loads depend on the program only, so an op's are
+    // issued while the op before it finishes (after its products, before
+    // its barrier); the op's HBM stores of the previous result always
+    // follow them, so no wait on them reaches back past those stores.
     float R[2][10];
     int kind[2];
     unsigned code[2];
     // An op with one child loads that child twice: every op issues the same
     // twenty loads, so the wait counts the compiler derives do not depend
     // on the path (a shorter path would lower them for all).
-    auto issueLoads = [&](const OpRec &op) {
-        ci4v *kp = kidsC + op.kidStart;
+    auto issueLoads = [&](const AaRec &op) {
 #pragma unroll
         for (int c = 0; c < 2; c++) {
-            const i4v kr = kp[c < op.nKids ? c : 0];
-            const KidRec kd{kr.x, kr.y, kr.z, kr.w};
+            const int cc = c < op.nKids ? c : 0;
+            const KidRec kd{op.kind[cc], op.idx[cc], op.pnode[cc], op.slot[cc]};
             const bool tip = kd.kind == 0;
             kind[c] = kd.kind;
             const unsigned cd = tipCode(tip ? kd.idx : 0);
             code[c] = cd;
-            const unsigned row = cd < (unsigned)AA_S ? cd * AA_S : 0u;
-            const unsigned o0 = (tip ? row + 4 * q : (unsigned)(q * AA_S + pl)) * 4u;
-            const unsigned o1 = (tip ? row + 15 : (unsigned)(q * AA_S + 16 + (pl & 3))) * 4u;
-            const int sb = tip ? 4 : 4 * AA_S * 4;   // bytes per step t
             const char *Pb = (const char *)PT + (size_t)(unsigned)(kd.pnode * K + k) * (AA_S * AA_S * 4);
+            if (tip) {
+                // two 16-byte loads per lane: a tip row touches one or two
+                // lines per pattern, and ten dword gathers would touch them
+                // five times over
+                const unsigned row = (cd < (unsigned)AA_S ? cd : 0u) * (AA_S * 4u);
+                const f4v v0 = *(const f4v *)(Pb + row + 16u * q);
+                const f4v v1 = *(const f4v *)(Pb + row + 64u);
+                for (int r = 0; r < 4; r++) {
+                    R[c][r] = v0[r];
+                    R[c][6 + r] = v1[r];
+                }
+            } else {
+                const unsigned o0 = (unsigned)(q * AA_S + pl) * 4u;
+                const unsigned o1 = (unsigned)(q * AA_S + 16 + (pl & 3)) * 4u;
 #pragma unroll
-            for (int t = 0; t < 5; t++) {
-                R[c][t] = *(const float *)(Pb + sb * t + o0);
-                R[c][5 + t] = *(const float *)(Pb + sb * t + o1);
+                for (int t = 0; t < 5; t++) {
+                    R[c][t] = *(const float *)(Pb + 320 * t + o0);
+                    R[c][5 + t] = *(const float *)(Pb + 320 * t + o1);
+                }
             }
         }
-    };
-    auto opAt = [&](int i) -> OpRec {
-        const i4v r = opsC[i];
-        return OpRec{r.x, r.y, r.z, r.w};
     };
 
     double contrib = 0.0;
@@ -1463,7 +1486,8 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
     // result overwrites one op later, in order, before anything reads it.
     // Programs here have two or more ops, the root last.
     const int nOps = a.nOps;
-    OpRec op = opAt(0);
+    int recNext = recLoad(1);
+    AaRec op = recTake(recLoad(0));
     pend = true;
     pDst = op.dst;
     pSrc = a.nLds;
@@ -1493,8 +1517,7 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                     const bool amb = cd >= (unsigned)AA_S && !all;
                     if (__builtin_amdgcn_ballot_w64(amb)) {
                         if (amb) {
-                            const KidRec kd = kidAt(op.kidStart + c);
-                            const float *Pk = PT + ((size_t)kd.pnode * K + k) * AA_S * AA_S;
+                            const float *Pk = PT + ((size_t)op.pnode[c] * K + k) * AA_S * AA_S;
                             const uint64_t m = a.ambMask[cd - AA_S];
                             v0 = zero;
                             v1 = zero;
@@ -1510,7 +1533,7 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
                 } else {
                     // (each branch reads through its own pointer: a pointer
                     // chosen between LDS and HBM would become a flat access)
-                    const KidRec kd = kidAt(op.kidStart + c);
+                    const KidRec kd{op.kind[c], op.idx[c], op.pnode[c], op.slot[c]};
                     f4v acc0 = zero, acc1 = zero;
                     // (the products sit in each branch, so the HBM branch's
                     // loads are consumed before the join and leave nothing
@@ -1597,11 +1620,12 @@ __device__ __forceinline__ void pruneAaCatBody(const PruneArgs &a, int tile, int
             if (q != 0) pr1 = zero;   // rows 4..15 of the states-16..19 products
             // the next op's loads go out now (R is free), so their latency
             // overlaps this op's barrier, rescale and result write
-            const OpRec cur = op;
-            if (i + 1 < nOps) {
-                op = opAt(i + 1);
-                issueLoads(op);
-            }
+            // (unconditional, past the end the last op again: every pass
+            // through the loop issues the same loads)
+            const AaRec cur = op;
+            op = recTake(recNext);
+            issueLoads(op);
+            recNext = recLoad(min(i + 2, nOps - 1));
             // rescale: max over this wave's states, then over the categories
             float m = 0.f;
             for (int r = 0; r < 4; r++) m = fmaxf(m, fmaxf(pr0[r], pr1[r]));
@@ -1727,7 +1751,8 @@ size_t pruneAaLdsBytes(int K, int nLds, int tipTaxa)
 }
 static bool aaLdsArgsOk(const PruneArgs &a)
 {
-    return a.nLds >= 0 && a.nLds <= pruneAaLdsSlots(a.K) && (!a.tipLds || pruneAaTipLds(a.nTax));
+    return a.nLds >= 0 && a.nLds <= pruneAaLdsSlots(a.K) && (!a.tipLds || pruneAaTipLds(a.nTax)) &&
+           a.aaRec != nullptr && a.nOps >= 2;
 }
 
 template <int KC>
