@@ -88,6 +88,43 @@ def test_partials_match_oracle(case, prec, tol):
         assert (numpy.abs(got - want) <= tol * scale).all(), n.nodeNum
 
 
+@pytest.mark.parametrize("cw,slots", [(False, 5), (True, 5), (True, 2), (True, 0)])
+def test_protein_ambiguity_partials_match_oracle(cw, slots):
+    """20-state tips with partial ambiguities (b = d|n, z = e|q), full ones
+    (x, gaps, '?') and pInvar 0.2, on a tree whose root has three children:
+    every stored partial against the oracle's, and lnL, for the tile kernel
+    and for category waves with 5, 2 and 0 LDS slots (0: every child read
+    back from HBM).  These take the kernels' rare paths (an ambiguous tip's
+    summed P rows, HBM children, a third root child)."""
+    fx = dict(load("prot128_150"))
+    rng = numpy.random.default_rng(11)
+    pat = fx["patterns"].copy()
+    hit = rng.random(pat.shape) < 0.12
+    # equate codes in the fixture's order (b, x, z), then gap and '?'
+    pat[hit] = rng.choice(numpy.array([-64, -62, -63, -1, -2]), size=int(hit.sum()))
+    fx["patterns"] = pat
+    fx["pInvar"] = numpy.array(0.2)
+    assert int((fx["parent"] == int(fx["root"])).sum()) == 3
+    f = oracle_flat(fx)
+    wantL, _, cl = oracle.oracle_cl(f)
+    K, S, nP = int(fx["nCat"]), int(fx["dim"]), f.patterns.shape[1]
+    try:
+        pf.setAaTuning(cw, 0)
+        pf.setAaLdsSlots(slots)
+        t = tree_from_fixture(fx, "fp32")
+        v = t.calcLogLike(verbose=0)
+        got = {n.nodeNum: pf.p4_getCondLikes(n.cNode, 0, K, S, nP) for n in t.nodes
+               if not (n.isLeaf or n is t.root)}
+    finally:
+        pf.setAaTuning(True)
+        pf.setAaLdsSlots(5)
+    assert rel(v, wantL) < FP32_TOL, (v, wantL)
+    for num, g in got.items():
+        want = cl[num]
+        scale = want.max(axis=(0, 1))[None, None, :]
+        assert (numpy.abs(g - want) <= 1e-4 * scale).all(), num
+
+
 # -------------------------------------------------- full-size workloads
 @pytest.mark.parametrize("name,nSites", [("gtr16", None), ("het64", None), ("prot128", 4000)])
 def test_workloads_vs_oracle(name, nSites):
