@@ -375,12 +375,111 @@ __device__ __forceinline__ void pexpBody(const PmatArgs &a, const double *in, in
         }
 }
 
+// The same P with 16 lanes per matrix, lane e holding entry (e / 4, e % 4):
+// a 4-state matrix's series is a chain of some 15 dependent products, and one
+// lane per matrix ran it at the latency of 64 serial FP64 FMAs per product.
+// Every lane forms Q, the scale and the doublings count for the whole matrix
+// exactly as pexpBody does (so all 16 agree bit for bit), keeps its row of
+// B, and fetches the columns (and, when doubling, the row) of the other
+// factor from the lanes of its matrix; each entry sums over m in matMul's
+// order, so P is pexpBody's bit for bit.  base: the matrix's first lane.
+__device__ __forceinline__ void pexpLanes4(const PmatArgs &a, const double *in, int it, int k, int e, int base)
+{
+    constexpr int S = 4;
+    const int r = e >> 2, c = e & 3;
+    const int node = a.nodeList[it];
+    const double t = a.t[(size_t)it * a.K + k];
+    const double *pi = in + (size_t)a.eigSlot[it] * (S + S * S);
+    const double *R = pi + S;
+    double B[S][S];
+    double sumODE = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < S; rr++) {
+        double row = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < S; cc++) {
+            const double q = (rr == cc) ? 0.0 : R[rr * S + cc] * pi[cc];
+            B[rr][cc] = q;
+            row += q;
+        }
+        B[rr][rr] = -row;
+        sumODE += pi[rr] * row;
+    }
+    const double f = t / sumODE;
+    double nrm = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < S; rr++) {
+        double row = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < S; cc++) {
+            B[rr][cc] *= f;
+            row += fabs(B[rr][cc]);
+        }
+        nrm = fmax(nrm, row);
+    }
+    int sq = 0;
+    if (nrm > 0.25) {
+        (void)frexp(nrm * 4.0, &sq);
+    }
+    double Br[S];   // this lane's row of B
+#pragma unroll
+    for (int m = 0; m < S; m++) Br[m] = ldexp(B[r][m], -sq);
+    const double diag = r == c ? 1.0 : 0.0;
+    double z = diag + Br[c] * (1.0 / PEXP_DEGREE);
+    auto col = [&](double v, double (&o)[S]) {
+#pragma unroll
+        for (int m = 0; m < S; m++) o[m] = __shfl(v, base + m * S + c, 64);
+    };
+#pragma unroll
+    for (int j = PEXP_DEGREE - 1; j >= 2; j--) {
+        double zc[S];
+        col(z, zc);
+        double acc = Br[0] * zc[0];
+#pragma unroll
+        for (int m = 1; m < S; m++) acc = fma(Br[m], zc[m], acc);
+        z = diag + acc * (1.0 / j);
+    }
+    double w;
+    {
+        double zc[S];
+        col(z, zc);
+        double acc = Br[0] * zc[0];
+#pragma unroll
+        for (int m = 1; m < S; m++) acc = fma(Br[m], zc[m], acc);
+        w = acc;   // E
+    }
+    for (int i = 0; i < sq; i++) {   // every lane of the matrix has the same sq
+        double wr[S], wc[S];
+#pragma unroll
+        for (int m = 0; m < S; m++) wr[m] = __shfl(w, base + r * S + m, 64);
+        col(w, wc);
+        double acc = wr[0] * wc[0];
+#pragma unroll
+        for (int m = 1; m < S; m++) acc = fma(wr[m], wc[m], acc);
+        w = fma(2.0, w, acc);
+    }
+    const size_t o = ((size_t)node * a.K + k) * S * S + (size_t)c * S + r;
+    const double pij = diag + w;
+    a.PT64[o] = pij;
+    a.PT32[o] = (float)pij;
+}
+
 template <int S>
 __global__ __launch_bounds__(64) void k_pexp(PmatArgs a, const double *in)
 {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= a.nItems * a.K) return;
     pexpBody<S>(a, in, gid / a.K, gid % a.K);
+}
+
+// 16 lanes per (branch, category): pexpLanes4.  A matrix's lanes share
+// one 16-lane row of the wave and leave together.
+__global__ __launch_bounds__(64) void k_pexpLanes4(PmatArgs a, const double *in)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mi = gid >> 4;
+    if (mi >= a.nItems * a.K) return;
+    pexpLanes4(a, in, mi / a.K, mi % a.K, gid & 15, (int)(threadIdx.x & 48));
 }
 
 // All items of a batched call in one launch: item i owns threads
@@ -394,6 +493,18 @@ __global__ __launch_bounds__(64) void k_pexpMulti(const PmatArgs *args, const do
     for (int j = threadIdx.x; j <= nItems; j += blockDim.x) sFirst[j] = first[j];
     __syncthreads();
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (S == 4) {
+        // 16 lanes per matrix (pexpLanes4)
+        const int mi = gid >> 4;
+        if (mi >= total) return;
+        int i = 0;
+        while (i + 1 < nItems && sFirst[i + 1] <= mi) i++;
+        const PmatArgs a = args[i];
+        const int local = mi - sFirst[i];
+        if (local >= a.nItems * a.K) return;
+        pexpLanes4(a, ins[i], local / a.K, local % a.K, gid & 15, (int)(threadIdx.x & 48));
+        return;
+    }
     if (gid >= total) return;
     int i = 0;
     while (i + 1 < nItems && sFirst[i + 1] <= gid) i++;
@@ -549,7 +660,8 @@ hipError_t launchPexpMulti(int dim, const PmatArgs *args, const double *const *i
     if (dim == 20)
         hipLaunchKernelGGL(k_pexpWaveMulti<20>, dim3(total), dim3(64), lds, s, args, ins, first, nItems);
     else if (dim == 4)
-        hipLaunchKernelGGL(k_pexpMulti<4>, dim3((total + 63) / 64), dim3(64), lds, s, args, ins, first, nItems, total);
+        hipLaunchKernelGGL(k_pexpMulti<4>, dim3(((size_t)total * 16 + 63) / 64), dim3(64), lds, s, args, ins, first,
+                           nItems, total);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -566,7 +678,7 @@ hipError_t launchPexp(const PmatArgs &a, const double *eigIn, hipStream_t s)
     const int total = a.nItems * a.K;
     if (total == 0) return hipSuccess;
     if (a.dim == 4) {
-        hipLaunchKernelGGL(k_pexp<4>, dim3((total + 63) / 64), dim3(64), 0, s, a, eigIn);
+        hipLaunchKernelGGL(k_pexpLanes4, dim3(((size_t)total * 16 + 63) / 64), dim3(64), 0, s, a, eigIn);
     } else if (a.dim == 20) {
         hipLaunchKernelGGL(k_pexpWave<20>, dim3(total), dim3(64), 0, s, a, eigIn);
     } else {
