@@ -630,7 +630,8 @@ static void stageParams(Tree *t, int pNum)
 // together with the other items' of the same state count.
 struct DeferredP {
     PmatArgs a;
-    const double *in;   // (pi, R) snapshots of the direct route; null: eigen route
+    const double *in;      // (pi, R) snapshots of the direct route (pinned arena); null: eigen route
+    const double *devIn;   // their device copy once the call's uploads have run, or null
 };
 
 // Error path of a batched call: the P flushes deferred so far are launched
@@ -649,7 +650,11 @@ static void launchDeferredEach(const std::vector<DeferredP> &defer)
     (void)hipStreamSynchronize(stream());
 }
 
-static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = nullptr)
+// devEig: the (pi, R) snapshots were copied to the device arena ahead of
+// the P kernel on the stream (20 states: one wavefront per (branch,
+// category) would otherwise read its 3.4 KB across PCIe, 1,012 times for
+// prot128)
+static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = nullptr, bool devEig = false)
 {
     TreePart &tp = t->tp[pNum];
     ModelPart *mp = t->model->parts[pNum];
@@ -668,13 +673,14 @@ static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = null
         PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
                    reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
                    reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
+        const double *inDev = devEig ? reinterpret_cast<const double *>(tp.dArena) : nullptr;
         if (direct && defer) {
-            defer->push_back({a, reinterpret_cast<const double *>(base)});
+            defer->push_back({a, reinterpret_cast<const double *>(base), inDev});
         } else if (direct) {
-            if (!hipOk(launchPexp(a, reinterpret_cast<const double *>(base), s), "launch P kernel"))
+            if (!hipOk(launchPexp(a, inDev ? inDev : reinterpret_cast<const double *>(base), s), "launch P kernel"))
                 return false;
         } else if (defer) {
-            defer->push_back({a, nullptr});
+            defer->push_back({a, nullptr, nullptr});
         } else if (!hipOk(launchPmat(a, s), "launch P kernel")) {
             return false;
         }
@@ -1589,6 +1595,11 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps, std::vector<Defer
     const size_t piBytes = sizeof(double) * mp->dim;
     if (!tp.rootPiOnDevice || std::memcmp(tp.rootPiUp, tp.hRootPi, piBytes) != 0) need(tp.offPi, tp.offPi + piBytes);
     if (!tp.progOnDevice) need(tp.offProg, tp.offProg + sizeof(int) * tp.progWords);
+    // 20-state P straight from exp(Qt) for many matrices: its (pi, R)
+    // snapshots (the arena's first bytes) go up in the same copy (a few
+    // matrices read them across PCIe sooner than a copy would run)
+    const bool devEig = tp.flushDirect && mp->dim == 20 && tp.pendI * mp->nCat >= 64;
+    if (devEig) need(0, sizeof(double) * (size_t)tp.nEig * (mp->dim + mp->dim * mp->dim));
     if (hi > lo) {
         if (uploads)
             uploads->push_back({tp.dArenaHost + lo, tp.dArena + lo, (unsigned long long)(hi - lo)});
@@ -1599,7 +1610,7 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps, std::vector<Defer
         tp.rootPiOnDevice = true;
         tp.progOnDevice = true;
     }
-    return launchParams(t, pNum, defer);
+    return launchParams(t, pNum, defer, devEig);
 }
 
 static void finishPart(Tree *t, int pNum, bool siteLikes, double lnL, bool bad)
@@ -3677,7 +3688,7 @@ int p4g_batchLogLike(p4g_batch *b, double *lnL)
             for (const DeferredP &dp : defer)
                 if (dp.in && dp.a.dim == dim) {
                     b->hPm[q] = dp.a;
-                    b->hPmIn[q] = dp.in;
+                    b->hPmIn[q] = dp.devIn ? dp.devIn : dp.in;   // the uploads ran before this launch
                     b->hPmFirst[f++] = total;
                     total += dp.a.nItems * dp.a.K;
                     q++;

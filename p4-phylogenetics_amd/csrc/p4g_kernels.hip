@@ -523,7 +523,7 @@ __device__ __forceinline__ void pexpWaveBody(const PmatArgs &a, const double *in
 {
     constexpr int NE = S * S;
     constexpr int PER = (NE + 63) / 64;
-    __shared__ double B[NE], Z[NE], W[NE];
+    __shared__ double B[NE], Z[NE], W[NE], T[NE];
     __shared__ double red[S];
     __shared__ double sumODE;
     const int lane = threadIdx.x;
@@ -571,18 +571,39 @@ __device__ __forceinline__ void pexpWaveBody(const PmatArgs &a, const double *in
         Z[e] = (r == c ? 1.0 : 0.0) + b * (1.0 / PEXP_DEGREE);
     }
     __syncthreads();
-    // acc[i] = (X Y)[entry lane + 64 i]
+    // acc[i] = (X Y)[entry lane + 64 i], on the FP64 matrix cores: the
+    // product padded to 32 x 32 is four 16 x 16 blocks, each five
+    // v_mfma_f64_16x16x4f64 steps over k (lane l: A[l % 16][l / 16],
+    // B[l / 16][l % 16]; result register r holds row l / 16 + 4 r of column
+    // l % 16), written to T and read back in the lane-owns-entries layout.
+    static_assert(S > 16 && S <= 32, "two 16-row blocks");
+    typedef double d4v __attribute__((ext_vector_type(4)));
+    const int q = lane >> 4, pl = lane & 15;
     auto product = [&](const double *X, const double *Y, double (&acc)[PER]) {
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++) {
+                const int i = rb * 16 + pl, j = cb * 16 + pl;
+                d4v d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < (S + 3) / 4; ks++) {
+                    const int kk = ks * 4 + q;
+                    const double av = (i < S && kk < S) ? X[i * S + kk] : 0.0;
+                    const double bv = (j < S && kk < S) ? Y[kk * S + j] : 0.0;
+                    d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = rb * 16 + q + 4 * r;   // the FP64 result layout interleaves rows
+                    if (row < S && j < S) T[row * S + j] = d[r];
+                }
+            }
+        __syncthreads();
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             const int e = lane + 64 * i;
-            acc[i] = 0.0;
-            if (e < NE) {
-                const int r = e / S, c = e - (e / S) * S;
-                double v = X[r * S] * Y[c];
-                for (int m = 1; m < S; m++) v = fma(X[r * S + m], Y[m * S + c], v);
-                acc[i] = v;
-            }
+            acc[i] = e < NE ? T[e] : 0.0;
         }
     };
     double acc[PER];
