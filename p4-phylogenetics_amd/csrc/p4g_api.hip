@@ -650,10 +650,10 @@ static void launchDeferredEach(const std::vector<DeferredP> &defer)
     (void)hipStreamSynchronize(stream());
 }
 
-// devEig: the (pi, R) snapshots were copied to the device arena ahead of
-// the P kernel on the stream (20 states: one wavefront per (branch,
-// category) would otherwise read its 3.4 KB across PCIe, 1,012 times for
-// prot128)
+// devEig: the (pi, R) snapshots and the pack were copied to the device
+// arena ahead of the P kernel on the stream (20 states: one wavefront per
+// (branch, category) would otherwise read its 3.4 KB across PCIe, 1,012
+// times for prot128, after two dependent reads of the pack)
 static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = nullptr, bool devEig = false)
 {
     TreePart &tp = t->tp[pNum];
@@ -669,7 +669,8 @@ static bool launchParams(Tree *t, int pNum, std::vector<DeferredP> *defer = null
     if (tp.pendI) {
         const int nI = tp.pendI;
         const bool direct = tp.flushDirect;
-        const char *d = base + tp.offPack;
+        // the pack (t, nodes, eigen slots) from the device copy too when it went up
+        const char *d = (devEig ? (const char *)tp.dArena : base) + tp.offPack;
         PmatArgs a{S, K, nI, reinterpret_cast<const int *>(d + sizeof(double) * nI * K),
                    reinterpret_cast<const int *>(d + sizeof(double) * nI * K + sizeof(int) * nI),
                    reinterpret_cast<const double *>(d), tp.eigV, tp.eigVi, tp.eigLam, tp.PT64, tp.PT32};
@@ -1596,10 +1597,10 @@ static bool stagePart(Tree *t, int pNum, bool full, int *nOps, std::vector<Defer
     if (!tp.rootPiOnDevice || std::memcmp(tp.rootPiUp, tp.hRootPi, piBytes) != 0) need(tp.offPi, tp.offPi + piBytes);
     if (!tp.progOnDevice) need(tp.offProg, tp.offProg + sizeof(int) * tp.progWords);
     // 20-state P straight from exp(Qt) for many matrices: its (pi, R)
-    // snapshots (the arena's first bytes) go up in the same copy (a few
-    // matrices read them across PCIe sooner than a copy would run)
+    // snapshots and the pack (the arena's first bytes) go up in the same
+    // copy (a few matrices read them across PCIe sooner than a copy runs)
     const bool devEig = tp.flushDirect && mp->dim == 20 && tp.pendI * mp->nCat >= 64;
-    if (devEig) need(0, sizeof(double) * (size_t)tp.nEig * (mp->dim + mp->dim * mp->dim));
+    if (devEig) need(0, tp.offPack + tp.packBytes);   // snapshots, eigen list and the pack
     if (hi > lo) {
         if (uploads)
             uploads->push_back({tp.dArenaHost + lo, tp.dArena + lo, (unsigned long long)(hi - lo)});
