@@ -1354,7 +1354,8 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     for (int i = 0; i < nOps; i++) nKidTot += ops[i].nKids;
     const size_t wordsW = (size_t)16 * nOps + 4 * (size_t)nSide + 3 * (size_t)nPre + (size_t)nKidTot;
     const size_t popsOff = (wordsW + 15) & ~(size_t)15;   // LeanOps, 64-byte aligned
-    const size_t tipPnOff = popsOff + (size_t)16 * nOps;
+    const int recInts = leanRecInts(t->precision == FP32);
+    const size_t tipPnOff = popsOff + (size_t)recInts * nOps;
     const size_t words = tipPnOff + (size_t)nStaged;
     if ((int)words > tp.progCapWords) {
         setError("internal: node program larger than its staging area");
@@ -1425,15 +1426,13 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
     {
         const size_t elt = t->precision == FP32 ? sizeof(float) : sizeof(double);
         const size_t blkB = 16 * elt;
-        LeanOp *lo = reinterpret_cast<LeanOp *>(tp.hProg + popsOff);
+        int *lo = tp.hProg + popsOff;
         int *tipPn = tp.hProg + tipPnOff;
         int nt = 0;
         for (int q = 0; q < nOps; q++) {
             const WaveOp &w = wo[q];
             LeanOp r{};
-            r.kind = PK_GEN;
-            r.gen = q;
-            r.ld = -1;
+            r.kindLd = PK_GEN;
             // tip children: staged P^T rows in LDS, or (no room to stage
             // them: FP64) scalar P^T and the select tree; both need the
             // tile's tip codes in LDS
@@ -1445,7 +1444,7 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
                 int c0 = 0, c1 = 1;
                 if (w.kind[0] != WK_TIP && w.kind[1] == WK_TIP) std::swap(c0, c1);   // tips first
                 const int nTip = (w.kind[0] == WK_TIP) + (w.kind[1] == WK_TIP);
-                r.kind = nTip == 2 ? PK_TT : nTip == 1 ? PK_TL : PK_LL;
+                const int kind = nTip == 2 ? PK_TT : nTip == 1 ? PK_TL : PK_LL;
                 const int cs[2] = {c0, c1};
                 int *aa[2] = {&r.a0, &r.a1}, *tt[2] = {&r.t0, &r.t1}, *pp[2] = {&r.p0, &r.p1};
                 for (int u = 0; u < 2; u++) {
@@ -1477,12 +1476,12 @@ static bool buildWaveProgram(Tree *t, int pNum, const std::vector<OpRec> &ops, c
                         *pp[u] = (int)pb;
                     }
                 }
-                const size_t de = (size_t)w.dst * tp.K * tp.nPatPad;
-                r.dstLo = (int)(uint32_t)(de & 0xffffffffu);
-                r.dstHi = (int)(uint32_t)(de >> 32);
-                r.ld = w.ldsDst >= 0 ? w.ldsDst : -1;
+                r.dst = w.dst;
+                r.kindLd = kind | ((w.ldsDst >= 0 ? w.ldsDst + 1 : 0) << 8);
             }
-            lo[q] = r;
+            int *rq = lo + (size_t)recInts * q;
+            std::memcpy(rq, &r, sizeof r);
+            for (int z = (int)(sizeof r / sizeof(int)); z < recInts; z++) rq[z] = 0;
         }
         // tips that only generic ops read are staged too (unused, harmless)
         for (; nt < nTipKid; nt++) tipPn[nt] = 0;
@@ -1512,6 +1511,7 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
     Part *dp = t->data->parts[pNum];
     const Node *root = t->nodes[t->root];
     const int *prog = reinterpret_cast<const int *>(tp.dArena + tp.offProg);
+    a.ptWarmBytes = 0;
     a.groups = reinterpret_cast<const GroupRec *>(prog);
     a.ops = reinterpret_cast<const OpRec *>(prog + tp.nGroups * 4);
     a.kids = reinterpret_cast<const KidRec *>(prog + (tp.nGroups + nOps) * 4);
@@ -1560,6 +1560,11 @@ static void fillPruneArgs(Tree *t, int pNum, int nOps, PruneArgs &a, bool siteLi
         a.pops = reinterpret_cast<const LeanOp *>(prog + tp.popsOff);
         a.nTipKid = tp.nTipKid;
         a.tipPn = prog + tp.tipPnOff;
+        // the P^T table, pulled into the L2 of every XCD the launch runs on
+        // while the tiles stage their tips (the scalar loads of the op chain
+        // then hit L2); skipped for tables that would crowd it
+        const size_t ptB = (size_t)t->nNodes * tp.K * 16 * (t->precision == FP32 ? sizeof(float) : sizeof(double));
+        a.ptWarmBytes = ptB <= LEAN_PT_WARM_MAX ? (int)ptB : 0;
     }
 }
 

@@ -182,22 +182,23 @@ struct WaveKid {
 // whose children are staged tips or LDS slots carry precomputed offsets, so
 // the kernel does no index arithmetic; every other op (the root,
 // multifurcations, children in global memory, unstaged tips) is PK_GEN and
-// runs from its WaveOp.  Tip children come first.
+// runs from its WaveOp (its record is all zeros but the kind).  Tip
+// children come first.  FP32 records are 32 bytes: the current and the next
+// record take 16 scalar registers, leaving room for both children's P^T
+// blocks; FP64 records are padded to 64 bytes (leanRecInts), which its
+// register allocation handles better.
 enum { PK_TT = 0, PK_TL = 1, PK_LL = 2, PK_GEN = 3 };
 struct LeanOp {
-    int kind;
+    int kindLd;          // PK_* | (LDS slot of the result + 1) << 8 (0: none)
     int a0, a1;          // tip: code offset row * 64; LDS child: slot index
     int t0, t1;          // tip: byte offset of its staged P^T block in the category's region;
                          // LDS child: exponent index slot * 64
     int p0, p1;          // byte offset of the child's P^T block in global memory, node * K * 16 * element
-    int dstLo, dstHi;    // result slot * K * nPatPad (in 4-state vectors)
-    int pad0, pad1;
-    int ld;              // LDS slot of the result, or -1
-    int pad2;
-    int gen;             // WaveOp index (PK_GEN)
-    int pad3, pad4;
+    int dst;             // partials slot of the result
 };
-static_assert(sizeof(LeanOp) == 64, "LeanOp is one 64-byte scalar load");
+static_assert(sizeof(LeanOp) == 32, "LeanOp is one 32-byte scalar load");
+inline int leanRecInts(bool fp32) { return fp32 ? 8 : 16; }   // record stride in ints
+constexpr size_t LEAN_PT_WARM_MAX = 64 * 1024;   // P^T tables up to this size are pulled into L2
 constexpr int WAVE_MAX_K = 8;             // categories (waves per workgroup)
 constexpr int WAVE_MAX_LDS_SLOTS = 16;
 constexpr int WAVE_TIP_LDS_MAX_TAXA = 256;
@@ -395,6 +396,7 @@ struct PruneArgs {
     int nTipKid;
     const int *tipPn;
     int nAmb;                // entries of ambMask
+    int ptWarmBytes;         // lean kernel: bytes of P^T to pull into L2 at the start (0: none)
     const int *aaRec;        // 20-state category kernel: fused op records, [nOps][AA_REC_WORDS]
 };
 

@@ -2395,7 +2395,7 @@ __global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneWaveBatch(const PruneA
 // category, workgroup = the K category waves of a 64-pattern tile) and
 // program, with three changes that cut the instructions per node op:
 //  * binary ops whose children are tips or LDS slots run from LeanOp
-//    records of precomputed byte offsets (one 64-byte scalar load);
+//    records of precomputed byte offsets (one 32-byte scalar load);
 //  * the P^T blocks of the program's tip children are staged in LDS at the
 //    start, so a tip child is one 16-byte LDS read of row `code` instead of
 //    a select tree over P held in scalar registers (16 moves + 12 selects);
@@ -2416,8 +2416,9 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
     typedef __attribute__((address_space(1))) const int gint;
     typedef __attribute__((address_space(1))) const unsigned char gu8;
     typedef __attribute__((address_space(4))) const char cchar;
-    typedef int rec16 __attribute__((ext_vector_type(16)));
-    typedef const __attribute__((address_space(4))) rec16 crec16;
+    // op records: 32 bytes in FP32, padded to 64 in FP64 (same fields)
+    typedef int rec8 __attribute__((ext_vector_type(sizeof(T) == 4 ? 8 : 16)));
+    typedef const __attribute__((address_space(4))) rec8 crec8;
     extern __shared__ __align__(16) unsigned char smemW[];
     __shared__ double rootL[WAVE_MAX_K][64];
     __shared__ int rootE[WAVE_MAX_K][64];
@@ -2441,6 +2442,14 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
                   lane;
     gu8 *gTips = (gu8 *)(a.tips);
     gint *gScl = (gint *)(a.scl);
+    // pull the P^T table and the op records into this XCD's L2 while the
+    // tile stages its tips: the op chain's scalar loads then hit L2 (the
+    // table was written by other XCDs' workgroups)
+    unsigned warm = 0;
+    for (int w = threadIdx.x * 128; w < a.ptWarmBytes; w += blockDim.x * 128)
+        warm ^= *reinterpret_cast<const unsigned *>(static_cast<const char *>(a.PT) + w);
+    for (int w = threadIdx.x * 128; w < a.nWOps * (int)sizeof(rec8); w += blockDim.x * 128)
+        warm ^= *reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(a.pops) + w);
     if (a.tipLds) {
         // the tile's tip codes, [taxon][64 patterns], 4 bytes per thread
         const int words = a.nTax * 16;
@@ -2521,12 +2530,13 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
         stS(l, ldG(o));
         if (SCALE) slotE[l * 64] = gScl[o];
     }
+    __asm__ volatile("" ::"v"(warm));   // the warming loads are kept and done here
     // no global load is left in flight past this point or past a generic
     // op (the fast ops issue none), so the fast path never waits on the
     // vector-memory counter, which also counts the partials' stores
     __builtin_amdgcn_s_waitcnt(VMCNT0);
 
-    crec16 *lops = (crec16 *)(a.pops);
+    crec8 *lops = (crec8 *)(a.pops);
     // the program's tip children read P^T rows staged in LDS, or (none
     // staged) scalar P^T through leafVec4's select tree, with the
     // partial-ambiguity masks from LDS (no global load in the fast loop)
@@ -2594,10 +2604,10 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
     // own loads before the fast loop resumes.
     int i = 0;
     while (true) {
-        rec16 rn = lops[i];   // each record is one 64-byte scalar load, one op ahead
+        rec8 rn = lops[i];   // each record is one scalar load, one op ahead
         for (; i < nW; i++) {
-            const rec16 r = rn;
-            const int kind = r[0];
+            const rec8 r = rn;
+            const int kind = r[0] & 0xff;
             if (kind == PK_GEN) break;
             if (i + 1 < nW) rn = lops[i + 1];
             T u[4], w[4], acc[4];
@@ -2650,7 +2660,7 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
 #pragma unroll
             for (int s = 0; s < 4; s++) acc[s] = u[s] * w[s];
             rescale(acc, e);
-            store(acc, e, (size_t)(unsigned)r[7] | ((size_t)(unsigned)r[8] << 32), r[11]);
+            store(acc, e, (size_t)(unsigned)r[7] * slotStride, (r[0] >> 8) - 1);
         }
         if (i >= nW) break;
         // ---- a generic op: any children, from anywhere; maybe the root
