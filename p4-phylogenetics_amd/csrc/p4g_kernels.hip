@@ -2583,15 +2583,27 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
             }
         }
     };
-    // the result to HBM and, when its parent is in the program, to LDS
+    // the result to HBM and, when its parent is in the program, to LDS.
+    // FP32: one 16-byte store per lane, so a wave's store covers whole
+    // lines; non-temporal, so the 190 MB of partials of a full evaluation
+    // stream past L2 instead of evicting the P^T table and op records the
+    // chain reads from it (het64: 67 us from 73).  FP64 results are two
+    // 16-byte stores per lane, each covering half of every line; streamed
+    // past L2 those cost twice the kernel's time, so they are cached.
     auto store = [&](const T *acc, int e, size_t dstE, int ld) {
         V4 rv;
         rv.x = acc[0];
         rv.y = acc[1];
         rv.z = acc[2];
         rv.w = acc[3];
-        clw[dstE + catOff] = rv;
-        if (SCALE) a.scl[dstE + catOff] = e;
+        if constexpr (sizeof(T) == 4) {
+            const tv4 nv = {acc[0], acc[1], acc[2], acc[3]};
+            __builtin_nontemporal_store(nv, reinterpret_cast<tv4 *>(a.cl) + dstE + catOff);
+            if (SCALE) __builtin_nontemporal_store(e, &a.scl[dstE + catOff]);
+        } else {
+            clw[dstE + catOff] = rv;
+            if (SCALE) a.scl[dstE + catOff] = e;
+        }
         if (ld >= 0) {
             stS(ld, rv);
             if (SCALE) slotE[ld * 64] = e;
