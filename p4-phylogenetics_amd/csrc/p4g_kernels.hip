@@ -2521,7 +2521,7 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
                 }
         }
     };
-    cchar *PTc = (cchar *)(a.PT) + (size_t)cat * blkB;
+    cchar *PTc = (cchar *)(a.PT) + (size_t)cat * blkB;   // + the records' unsigned P^T byte offsets (scalar loads with a register offset)
 
     const int *pre = a.wpre;
     for (int i = 0; i < a.nPre; i++) {
@@ -2616,6 +2616,9 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
     // own loads before the fast loop resumes.
     int i = 0;
     while (true) {
+        // FP64: the op counter held in a scalar register (het64 100 us from
+        // 103-109; in FP32 the same costs 2 us)
+        if constexpr (sizeof(T) == 8) i = __builtin_amdgcn_readfirstlane(i);
         rec8 rn = lops[i];   // each record is one scalar load, one op ahead
         for (; i < nW; i++) {
             const rec8 r = rn;
@@ -2627,8 +2630,8 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
             if (kind == PK_LL) {
                 const V4 x = ldS(r[1]), y = ldS(r[2]);
                 if (SCALE) e = slotE[r[3]] + slotE[r[4]];
-                cT *P0 = (cT *)(PTc + r[5]);
-                cT *P1 = (cT *)(PTc + r[6]);
+                cT *P0 = (cT *)(PTc + (unsigned)r[5]);
+                cT *P1 = (cT *)(PTc + (unsigned)r[6]);
                 T Pa[16], Pb[16];
 #pragma unroll
                 for (int j = 0; j < 16; j++) {
@@ -2642,7 +2645,7 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
                 if (stagedTips) {
                     tipRow(r[3], (unsigned)tipCodes[r[1]], u);
                 } else {
-                    cT *Pt = (cT *)(PTc + r[5]);
+                    cT *Pt = (cT *)(PTc + (unsigned)r[5]);
                     T Pl[16];
 #pragma unroll
                     for (int j = 0; j < 16; j++) Pl[j] = Pt[j];
@@ -2652,7 +2655,7 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
                     if (stagedTips) {
                         tipRow(r[4], (unsigned)tipCodes[r[2]], w);
                     } else {
-                        cT *Pt = (cT *)(PTc + r[6]);
+                        cT *Pt = (cT *)(PTc + (unsigned)r[6]);
                         T Pl[16];
 #pragma unroll
                         for (int j = 0; j < 16; j++) Pl[j] = Pt[j];
@@ -2661,7 +2664,7 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
                 } else {
                     const V4 y = ldS(r[2]);
                     if (SCALE) e = slotE[r[4]];
-                    cT *P1 = (cT *)(PTc + r[6]);
+                    cT *P1 = (cT *)(PTc + (unsigned)r[6]);
                     T Pb[16];
 #pragma unroll
                     for (int j = 0; j < 16; j++) Pb[j] = P1[j];
