@@ -2399,8 +2399,9 @@ __global__ __launch_bounds__(64 * WAVE_MAX_K) void k_pruneWaveBatch(const PruneA
 //  * the P^T blocks of the program's tip children are staged in LDS at the
 //    start, so a tip child is one 16-byte LDS read of row `code` instead of
 //    a select tree over P held in scalar registers (16 moves + 12 selects);
-//  * an internal child's P^T comes by one 64-byte scalar load per child and
-//    feeds the 16 FMAs directly as scalar operands.
+//  * an internal child's P^T comes by scalar loads (one 64-byte load per
+//    child in FP32, two in FP64) and feeds the 16 FMAs directly as scalar
+//    operands.
 // The root, multifurcations and children in global memory take the generic
 // path of the wave kernel.
 template <typename T, bool SCALE>
