@@ -2418,8 +2418,8 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
     typedef __attribute__((address_space(1))) const unsigned char gu8;
     typedef __attribute__((address_space(4))) const char cchar;
     // op records: 32 bytes in FP32, padded to 64 in FP64 (same fields)
-    typedef int rec8 __attribute__((ext_vector_type(sizeof(T) == 4 ? 8 : 16)));
-    typedef const __attribute__((address_space(4))) rec8 crec8;
+    typedef int LeanRecV __attribute__((ext_vector_type(sizeof(T) == 4 ? 8 : 16)));
+    typedef const __attribute__((address_space(4))) LeanRecV cLeanRecV;
     extern __shared__ __align__(16) unsigned char smemW[];
     __shared__ double rootL[WAVE_MAX_K][64];
     __shared__ int rootE[WAVE_MAX_K][64];
@@ -2449,7 +2449,7 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
     unsigned warm = 0;
     for (int w = threadIdx.x * 128; w < a.ptWarmBytes; w += blockDim.x * 128)
         warm ^= *reinterpret_cast<const unsigned *>(static_cast<const char *>(a.PT) + w);
-    for (int w = threadIdx.x * 128; w < a.nWOps * (int)sizeof(rec8); w += blockDim.x * 128)
+    for (int w = threadIdx.x * 128; w < a.nWOps * (int)sizeof(LeanRecV); w += blockDim.x * 128)
         warm ^= *reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(a.pops) + w);
     if (a.tipLds) {
         // the tile's tip codes, [taxon][64 patterns], 4 bytes per thread
@@ -2522,7 +2522,9 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
                 }
         }
     };
-    cchar *PTc = (cchar *)(a.PT) + (size_t)cat * blkB;   // + the records' unsigned P^T byte offsets (scalar loads with a register offset)
+    // this category's P^T blocks; a record's P^T byte offsets are added as
+    // unsigned values, which the scalar loads take as a register offset
+    cchar *PTc = (cchar *)(a.PT) + (size_t)cat * blkB;
 
     const int *pre = a.wpre;
     for (int i = 0; i < a.nPre; i++) {
@@ -2537,7 +2539,7 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
     // vector-memory counter, which also counts the partials' stores
     __builtin_amdgcn_s_waitcnt(VMCNT0);
 
-    crec8 *lops = (crec8 *)(a.pops);
+    cLeanRecV *lops = (cLeanRecV *)(a.pops);
     // the program's tip children read P^T rows staged in LDS, or (none
     // staged) scalar P^T through leafVec4's select tree, with the
     // partial-ambiguity masks from LDS (no global load in the fast loop)
@@ -2620,9 +2622,9 @@ __device__ __forceinline__ void pruneLeanBody(const PruneArgs &a, int tile, int 
         // FP64: the op counter held in a scalar register (het64 100 us from
         // 103-109; in FP32 the same costs 2 us)
         if constexpr (sizeof(T) == 8) i = __builtin_amdgcn_readfirstlane(i);
-        rec8 rn = lops[i];   // each record is one scalar load, one op ahead
+        LeanRecV rn = lops[i];   // each record is one scalar load, one op ahead
         for (; i < nW; i++) {
-            const rec8 r = rn;
+            const LeanRecV r = rn;
             const int kind = r[0] & 0xff;
             if (kind == PK_GEN) break;
             if (i + 1 < nW) rn = lops[i + 1];
